@@ -1,0 +1,221 @@
+/*
+ * meso_hip.h — C ABI of the MI355X (gfx950) DUGKS/CDUGKS time step.
+ *
+ * The step replaces meso's per-step update, i.e. the body of
+ *   CDUGKS_SHAKHOV::do_step()  src/solver/boltzmann/cdugks_shakhov.cpp:724-764
+ *   CDUGKS::do_step()          src/solver/boltzmann/cdugks.cpp:337-371
+ *   DUGKS::do_step()           src/solver/boltzmann/dugks.cpp:346-383
+ * (reconstruct() + fvm_update() + residual bookkeeping), called from the
+ * time loop of handle_solver<T>() (src/meso/exec.cpp:81-165).
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a negative MESO_E_* code on
+ *     failure; meso_hip_last_error()/meso_case_last_error() give the text.
+ *     No C++ exception crosses this boundary.
+ *   - Host arrays passed in are borrowed for the duration of the call only.
+ *   - A context owns its device memory and one HIP stream (default priority);
+ *     calls on one context are stream-ordered and not thread-safe.
+ *   - Distributions on the host are FP64. Layout VELOCITY_MAJOR is meso's own
+ *     DistributionFunction layout f[p][cell] (include/solver/solver.h:55);
+ *     CELL_MAJOR is f[cell][p]. For two distributions (Shakhov g, h) the
+ *     buffer holds all of g followed by all of h.
+ */
+#ifndef MESO_HIP_H
+#define MESO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MESO_OK 0
+#define MESO_E_INVALID (-1)  /* bad argument / unsupported case        */
+#define MESO_E_IO (-2)       /* file could not be read or parsed       */
+#define MESO_E_DEVICE (-3)   /* HIP runtime error                      */
+#define MESO_E_NUMERIC (-4)  /* NaN density or T<0 detected in a step  */
+
+/* ---- enumerations ------------------------------------------------------ */
+/* solver selected by the case key `solver` (src/meso/main.cpp:47-57) */
+#define MESO_MODEL_CDUGKS_BGK 1     /* "cdugks@incompressible" */
+#define MESO_MODEL_CDUGKS_SHAKHOV 2 /* "cdugks@shakhov"        */
+#define MESO_MODEL_DUGKS_BGK 3      /* "dugks@incompressible"  */
+
+#define MESO_STORAGE_FP64 0 /* f, fluxes, gradients in FP64 */
+#define MESO_STORAGE_FP32 1 /* f, fluxes, gradients in FP32; moments, macros, residual in FP64 */
+
+#define MESO_LAYOUT_VELOCITY_MAJOR 0
+#define MESO_LAYOUT_CELL_MAJOR 1
+
+/* Boundary kinds: the values of meso's BoundaryType (include/solver/config.h:8-18). */
+#define MESO_BC_FLUID_INTERIOR 0
+#define MESO_BC_FARFIELD_INLET 1
+#define MESO_BC_PRESSURE_INLET 2
+#define MESO_BC_FREESTREAM_INLET 3
+#define MESO_BC_FARFIELD_OUTLET 4
+#define MESO_BC_PRESSURE_OUTLET 5
+#define MESO_BC_WALL 6
+#define MESO_BC_SLIP_WALL 7
+#define MESO_BC_SYMMETRY 8
+
+/* Patch kinds of a mark's "temperature" key (include/solver/config.h:20-30). */
+#define MESO_PATCH_ZERO_GRADIENT 0
+#define MESO_PATCH_FIXED_VALUE 1
+
+/* ---- plain-data descriptors -------------------------------------------- */
+
+/* One [mark] block of the case file (include/solver/config.h:66-72). */
+typedef struct meso_mark_desc {
+    int32_t type;      /* MESO_BC_*                                    */
+    int32_t T_kind;    /* MESO_PATCH_* of key "temperature"            */
+    int32_t direction; /* mirror axis (0,1,2) for symmetry / slip-wall */
+    int32_t reserved;
+    double velocity[3];
+    double temperature;
+    double density;
+    double pressure;
+    double alpha_u; /* slip-wall-alpha-u */
+    double alpha_T; /* slip-wall-alpha-T */
+} meso_mark_desc;
+
+/* Mesh topology + geometry as built by fvmMesh::load_gambit
+ * (src/mesh/object.cpp:38-172, src/mesh/geom.cpp:165-211). */
+typedef struct meso_mesh_desc {
+    int32_t dim;
+    int32_t n_cells;
+    int32_t n_faces;
+    int32_t n_marks;
+    int32_t n_nbr;        /* entries of the neighbour CSR      */
+    int32_t n_cell_faces; /* entries of the cell->face CSR     */
+    double total_volume;
+    double min_cell_size;
+    const double *cell_pos;       /* [n_cells][3] node-average centroid        */
+    const double *cell_vol;       /* [n_cells]                                 */
+    const int32_t *face_cells;    /* [n_faces][2] owner, neighbour (=owner on a boundary) */
+    const double *face_pos;       /* [n_faces][3] node-average face centre     */
+    const double *face_area;      /* [n_faces]                                 */
+    const double *face_normal;    /* [n_faces][3] unit normal out of the owner */
+    const int32_t *cell_face_ptr; /* [n_cells+1]                               */
+    const int32_t *cell_face_ids; /* [n_cell_faces] faces of each cell, local order */
+    const int32_t *nbr_ptr;       /* [n_cells+1] least-squares neighbours      */
+    const int32_t *nbr_ids;       /* [n_nbr]                                   */
+    const double *lsq_dr;         /* [n_nbr][3] x_nbr - x_cell                 */
+    const double *lsq_w;          /* [n_nbr] 1/|dr|^2                          */
+    const double *lsq_C;          /* [n_cells][9] rows Cx, Cy, Cz              */
+    const int32_t *face_mark;     /* [n_faces] index into marks (marks[0] = fluid-interior) */
+    const meso_mark_desc *marks;  /* [n_marks]                                 */
+} meso_mesh_desc;
+
+/* Discrete velocity set (src/solver/dvs/dvs_generator.cpp). */
+typedef struct meso_dvs_desc {
+    int32_t n_vel;
+    int32_t has_mirror;
+    double max_magnitude;
+    const double *xi;      /* [n_vel][3] */
+    const double *w;       /* [n_vel]    */
+    const int32_t *mirror; /* [n_vel][3] mirror velocity per axis, or NULL */
+} meso_dvs_desc;
+
+/* Step parameters (CDUGKS_SHAKHOV::update_config cdugks_shakhov.cpp:7-33,
+ * CDUGKS ctor/initial cdugks.cpp:9-18,45-47, DUGKS initial dugks.cpp:45-48). */
+typedef struct meso_step_params {
+    int32_t model;           /* MESO_MODEL_*              */
+    int32_t dim;             /* 2 or 3                    */
+    int32_t gradient_switch; /* 0: zero-gradient faces    */
+    int32_t K;               /* gas-k (internal DOF), Shakhov */
+    double dt;
+    /* Shakhov */
+    double Pr, R, T0, mu0, omega;
+    /* incompressible BGK */
+    double RT, tau;
+} meso_step_params;
+
+/* Result of meso_hip_step(). */
+typedef struct meso_step_report {
+    int32_t steps_done;
+    int32_t bad_cells;   /* cells with NaN density or T<0 / NaN T in the last step */
+    int32_t first_bad_cell;
+    int32_t reserved;
+} meso_step_report;
+
+/* ======================================================================== */
+/* Host front-end: case file -> descriptors                                   */
+/* ======================================================================== */
+typedef struct meso_case meso_case;
+
+/* Parse a meso case file, its gambit mesh and its DVS
+ * (Config ctor src/solver/config.cpp:18-28; solver ctors cdugks_shakhov.cpp:35-60,
+ * cdugks.cpp:7-39, dugks.cpp:7-39). mesh_scale_override <= 0 keeps `mesh-scale`. */
+int meso_case_open(const char *case_path, double mesh_scale_override, meso_case **out);
+void meso_case_close(meso_case *c);
+const char *meso_case_last_error(void);
+
+/* Descriptors point into memory owned by the case; valid until meso_case_close. */
+int meso_case_mesh(const meso_case *c, meso_mesh_desc *out);
+int meso_case_dvs(const meso_case *c, meso_dvs_desc *out);
+int meso_case_params(const meso_case *c, meso_step_params *out);
+/* Re-read [settings] (dt, Pr, mu0, ...) like update_config() does each step. */
+int meso_case_reload_settings(meso_case *c);
+/* Per-cell initial density, velocity, temperature from the cell-group patches
+ * (fixedValue or fromFile; cdugks_shakhov.cpp:108-131, cdugks.cpp:69-84). */
+int meso_case_initial_patch(const meso_case *c, double *rho, double *u3, double *T);
+/* Integer / string settings the driver loop needs. */
+int meso_case_setting_int(const meso_case *c, const char *key, int32_t dflt, int32_t *out);
+int meso_case_setting_double(const meso_case *c, const char *key, double dflt, double *out);
+/* Writes at most cap bytes (NUL included). */
+int meso_case_setting_str(const meso_case *c, const char *key, const char *dflt, char *out, int32_t cap);
+
+/* ======================================================================== */
+/* Device step                                                               */
+/* ======================================================================== */
+typedef struct meso_hip_ctx meso_hip_ctx;
+
+/* Upload the mesh, DVS and parameters; allocate the state on the current device. */
+int meso_hip_create(const meso_mesh_desc *mesh, const meso_dvs_desc *dvs, const meso_step_params *params,
+                    int32_t storage, meso_hip_ctx **out);
+void meso_hip_destroy(meso_hip_ctx *ctx);
+const char *meso_hip_last_error(const meso_hip_ctx *ctx);
+
+/* initial(): f = equilibrium of the per-cell patch values; macros from the
+ * discrete moments; Shakhov q = m3/2 (cdugks_shakhov.cpp:103-168,
+ * cdugks.cpp:67-104, dugks.cpp:68-105). Also resets the residual copies. */
+int meso_hip_init_equilibrium(meso_hip_ctx *ctx, const double *rho, const double *u3, const double *T);
+/* read-np-data restart: f = equilibrium (Shakhov: g_S/h_S with q) of the given
+ * macros, which are taken as the cell state (cdugks_shakhov.cpp:81-101, cdugks.cpp:56-65). */
+int meso_hip_init_from_macros(meso_hip_ctx *ctx, const double *rho, const double *u3, const double *T,
+                              const double *q3);
+/* Full state upload / download (f in FP64 host layout, macros per cell). T and
+ * q3 are ignored (may be NULL) for the BGK models. */
+int meso_hip_set_state(meso_hip_ctx *ctx, const double *f, int32_t layout, const double *rho, const double *u3,
+                       const double *T, const double *q3);
+int meso_hip_get_state(meso_hip_ctx *ctx, double *f, int32_t layout);
+
+/* Advance nsteps time steps. Stops early (returning MESO_E_NUMERIC) after a
+ * step that produced NaN density or T<0 (cdugks_shakhov.cpp:698-719). */
+int meso_hip_step(meso_hip_ctx *ctx, int32_t nsteps, meso_step_report *report);
+int meso_hip_set_params(meso_hip_ctx *ctx, const meso_step_params *params);
+
+/* Cell macros after the last step: rho[NC], u3[NC*3], T[NC], q3[NC*3] (NULL skips). */
+int meso_hip_get_macros(meso_hip_ctx *ctx, double *rho, double *u3, double *T, double *q3);
+/* Cell flux moments of the last step (fluxM0..2 of cdugks_shakhov.cpp:837-839). */
+int meso_hip_get_flux_moments(meso_hip_ctx *ctx, double *m0, double *m1_3, double *m2);
+/* Residuals against the stored previous copies, then copy (field.cpp:222-249):
+ * out[0]=rho, out[1..3]=u, out[4]=T, out[5..7]=q (BGK: T and q entries are 0). */
+int meso_hip_residual(meso_hip_ctx *ctx, double out[8]);
+
+/* Kernel timing with HIP events on the context's stream. enable=1 starts
+ * recording (and clears totals), enable=0 stops. Names are static strings. */
+int meso_hip_profile(meso_hip_ctx *ctx, int32_t enable);
+int meso_hip_kernel_times(meso_hip_ctx *ctx, int32_t max_entries, const char **names, double *total_ms,
+                          int32_t *launches, int32_t *n_entries);
+/* Bytes of device memory held by the context. */
+int64_t meso_hip_device_bytes(const meso_hip_ctx *ctx);
+/* Synchronise the context's stream. */
+int meso_hip_sync(meso_hip_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MESO_HIP_H */

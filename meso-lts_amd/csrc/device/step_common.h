@@ -1,0 +1,160 @@
+// Device-side building blocks of the gfx950 DUGKS/CDUGKS step.
+//
+// Layout in HBM: every distribution is cell-major, f[c * nvp + v], the
+// velocity index fastest so that the lanes of a wave (lanes along v) read and
+// write contiguous 256 B (FP32) / 512 B (FP64) segments of one cell row.
+// nvp is nv rounded up to 64 when nv >= 64 (rows start on 256 B boundaries);
+// padded entries stay zero and are never read as velocities.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace meso_dev {
+
+constexpr double VSMALL = 1e-30;  // upwind dead-band, include/core/math.h:4
+constexpr double PI = 3.14159265358979323846;
+
+enum Model : int { CDUGKS_BGK = 1, CDUGKS_SHAKHOV = 2, DUGKS_BGK = 3 };
+
+// Boundary kinds = meso BoundaryType values.
+enum Bc : int {
+    BC_INTERIOR = 0,
+    BC_FARFIELD_INLET = 1,
+    BC_PRESSURE_INLET = 2,
+    BC_FREESTREAM_INLET = 3,
+    BC_FARFIELD_OUTLET = 4,
+    BC_PRESSURE_OUTLET = 5,
+    BC_WALL = 6,
+    BC_SLIP_WALL = 7,
+    BC_SYMMETRY = 8,
+};
+
+// Per-face equilibrium description, written by the face-macro kernel.
+// Shakhov: g_S = A exp(-B|c|^2) [1 + P1 (c.q)(|c|^2 iRT - D - 2)]
+//          h_S = g_M [hk + P2 (c.q)((|c|^2 iRT - D)(K+3-D) - 2K)],  c = xi - u
+// BGK:     f_eq = rho (1 + xi.u iRT + ((xi.u)^2 iRT^2 - |u|^2 iRT)/2), A = rho
+struct EqParams {
+    double u[3];
+    double q[3];
+    double A, B, iRT, P1, P2, hk;
+    double mix;  // weight of the equilibrium in the face value (C_s or c_eq)
+    double pad;
+};
+
+// Parameters of one boundary face after the wall / inlet / outlet stage: the
+// value imposed on incoming velocities is the equilibrium `in` (Maxwellian
+// of rho_b, u_b, T_b; BGK polynomial), optionally mixed with the mirror
+// velocity's value.
+struct BndParams {
+    EqParams in;
+    int kind;       // Bc
+    int dir;        // mirror axis
+    double alpha_u; // weight of the mirror value for g (slip-wall)
+    double alpha_T; // weight of the mirror value for h (slip-wall)
+    double pad;
+};
+
+struct StepConst {
+    int model, D, K, grad;
+    int nc, nf, nv, nvp;
+    int nb;            // boundary faces
+    int ntile_f, vt_f; // velocity tiles of the face-moment pass
+    int ntile_c, vt_c; // velocity tiles of the cell pass
+    double dt, half_dt;
+    double Pr, R, T0, mu0, omega, Cv, gamma;
+    double RT, tau;    // BGK
+};
+
+template <class R>
+__device__ __forceinline__ R dexp(R x);
+template <>
+__device__ __forceinline__ double dexp<double>(double x) { return exp(x); }
+template <>
+__device__ __forceinline__ float dexp<float>(float x) { return expf(x); }
+
+// Shakhov equilibrium pair at velocity xi for face/cell constants e.
+template <class R>
+__device__ __forceinline__ void shakhov_eq(const EqParams &e, const R xi[3], int D, int K, R &gs, R &hs) {
+    R c0 = xi[0] - R(e.u[0]), c1 = xi[1] - R(e.u[1]), c2 = xi[2] - R(e.u[2]);
+    R cc = c0 * c0 + c1 * c1 + c2 * c2;
+    R cq = c0 * R(e.q[0]) + c1 * R(e.q[1]) + c2 * R(e.q[2]);
+    R gm = R(e.A) * dexp<R>(-cc * R(e.B));
+    R ccr = cc * R(e.iRT);
+    gs = gm * (R(1) + R(e.P1) * cq * (ccr - R(D) - R(2)));
+    hs = gm * (R(e.hk) + R(e.P2) * cq * ((ccr - R(D)) * R(K + 3 - D) - R(2 * K)));
+}
+
+// Incompressible BGK polynomial equilibrium (cdugks.cpp:116-121).
+template <class R>
+__device__ __forceinline__ R bgk_eq(const EqParams &e, const R xi[3]) {
+    R ku = (xi[0] * R(e.u[0]) + xi[1] * R(e.u[1]) + xi[2] * R(e.u[2])) * R(e.iRT);
+    R uu = (R(e.u[0]) * R(e.u[0]) + R(e.u[1]) * R(e.u[1]) + R(e.u[2]) * R(e.u[2])) * R(e.iRT);
+    return R(e.A) * (R(1) + ku + (ku * ku - uu) * R(0.5));
+}
+
+// Host/device: constants of the Shakhov equilibrium of (rho, u, T, q).
+__host__ __device__ inline void make_shakhov(EqParams &e, double rho, const double u[3], double T,
+                                             const double q[3], double R, double Pr, int D, int K) {
+    double RT = R * T;
+    double p = rho * RT;
+    for (int k = 0; k < 3; ++k) {
+        e.u[k] = u[k];
+        e.q[k] = q[k];
+    }
+    if (D == 2) {
+        double over_RT2 = 0.5 / RT;
+        e.A = rho * over_RT2 / PI;
+        e.B = over_RT2;
+    } else {
+        e.A = rho / pow(2.0 * PI * RT, 1.5);
+        e.B = 0.5 / RT;
+    }
+    e.iRT = 1.0 / RT;
+    e.P1 = (1.0 - Pr) / (5.0 * p * RT);
+    e.P2 = (1.0 - Pr) / (5.0 * p);
+    e.hk = (K + 3.0 - D) * RT;
+    e.mix = 0.0;
+    e.pad = 0.0;
+}
+
+__host__ __device__ inline void make_bgk(EqParams &e, double rho, const double u[3], double RT) {
+    for (int k = 0; k < 3; ++k) {
+        e.u[k] = u[k];
+        e.q[k] = 0.0;
+    }
+    e.A = rho;
+    e.B = 0.0;
+    e.iRT = 1.0 / RT;
+    e.P1 = e.P2 = e.hk = 0.0;
+    e.mix = 0.0;
+    e.pad = 0.0;
+}
+
+// tau(rho, T) = mu0 (T/T0)^omega / (rho R T)   (cdugks_shakhov.cpp:181-183)
+__host__ __device__ inline double tau_of(const StepConst &s, double rho, double T) {
+    return s.mu0 * pow(T / s.T0, s.omega) / (rho * s.R * T);
+}
+
+// Sum over the LP lanes of one segment (LP a power of two <= 64).
+template <int LP>
+__device__ __forceinline__ double seg_sum(double x) {
+#pragma unroll
+    for (int o = LP / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+// Logical block id that gives each XCD a contiguous range of block ids:
+// hardware dispatches block b to XCD b % 8, so consecutive blocks of one
+// XCD get consecutive logical ids and share cell rows in that XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nblocks) {
+    constexpr int NX = 8;
+    int per = (nblocks + NX - 1) / NX;
+    int x = b % NX, k = b / NX;
+    int full = nblocks - (per - 1) * NX;  // XCDs that receive `per` blocks
+    int base = (x < full) ? x * per : full * per + (x - full) * (per - 1);
+    return base + k;
+}
+
+}  // namespace meso_dev

@@ -1,0 +1,158 @@
+// gfx950 kernels of one DUGKS/CDUGKS time step.
+//
+// One step is five launches (six for classic DUGKS), all on the context's
+// stream:
+//   k_face_moments  (face, v-tile) segments: half-step reconstruction f-bar+ of
+//                   every face from its upwind cell and that cell's LSQ
+//                   gradient, and the raw velocity moments of f-bar+ summed in
+//                   FP64 per tile  (cdugks_shakhov.cpp:213-316)
+//   k_face_eq       per face: moments -> rho, u, T, q, tau, C_s; equilibrium
+//                   constants; FP64 flux moments of the mixed face value along
+//                   n0 (cdugks_shakhov.cpp:317-362 and the face part of :616-641)
+//   k_boundary      per boundary face (one block, all v): wall density sums,
+//                   inlet/outlet/wall/slip/symmetry values, exact flux moments
+//                   (cdugks_shakhov.cpp:364-607)
+//   k_cell_update   (cell, v-tile) segments: conservative macro update from
+//                   the face flux moments, face values recomputed from the
+//                   2-ring of f^n, cell flux, f-tilde-plus update, q partials
+//                   (cdugks_shakhov.cpp:610-722)
+//   k_cell_q        per cell: q^{n+1} from the partials (cdugks_shakhov.cpp:721)
+//
+// Nothing of size NF x NV is ever stored: face values are recomputed from cell
+// values wherever they are needed, so a step streams f^n twice and writes
+// f^{n+1} once.
+#pragma once
+
+#include "step_common.h"
+
+namespace meso_dev {
+
+// Everything a kernel needs; passed by value.
+struct KArgs {
+    StepConst s;
+    // mesh
+    const int2 *face_cells;     // [nf] owner, neighbour
+    const double *face_n;       // [nf*3] unit normal out of the owner
+    const double *face_d;       // [nf*6] x_f - x_owner, x_f - x_neighbour
+    const float *face_df;       // float copy of face_d
+    const int *cf_ptr;          // [nc+1] cell -> faces
+    const int *cf_ids;          // faces
+    const double *cf_sA;        // +-area (+ when the cell owns the face)
+    const int *nb_ptr;          // [nc+1] LSQ neighbours
+    const int *nb_ids;
+    const double *nb_a;         // [nnz*3] w_j (Cx.dr_j, Cy.dr_j, Cz.dr_j)
+    const float *nb_af;
+    const double *cell_vol;
+    const int *face_bnd;        // [nf] boundary index or -1
+    const int *bnd_face;        // [nb]
+    const int *bnd_mark;        // [nb] index into marks
+    const void *marks;          // meso_mark_desc[] (host layout)
+    // dvs
+    const double *xi;           // [nv*3]
+    const float *xif;
+    const double *w;            // [nv]
+    const int *mirror;          // [nv*3] or null
+    // state
+    const void *fin_g, *fin_h;  // cell-major, f^n
+    void *fout_g, *fout_h;      // cell-major, f^{n+1}
+    const double *rho_in, *u_in, *T_in, *q_in;
+    double *rho_out, *u_out, *T_out, *q_out;
+    // scratch
+    double *part_f;             // [nf][ntile_f][ncol]
+    EqParams *face_eq;          // [nf]
+    double *face_flux;          // [nf][D+2] flux moments along n0 of the final face value
+    BndParams *bnd;             // [nb]
+    double *part_c;             // [nc][ntile_c][4]
+    double *fm;                 // [nc][5] cell flux moments: m0, m1 (3), m2
+    // control
+    int *bad_count;
+    int *first_bad;
+    int *bad_step;
+    int *halt;
+    int step_index;
+};
+
+template <int MODEL, int D>
+__host__ __device__ constexpr int ncol_face() {
+    // g: M0, M1 (D), M2 (D(D+1)/2), [M3 (D), H0, H1 (D) for Shakhov]
+    return MODEL == CDUGKS_SHAKHOV ? 1 + D + D * (D + 1) / 2 + D + 1 + D : 1 + D + D * (D + 1) / 2;
+}
+
+// LSQ gradient of distribution f at cell c, velocity v (mesh_numerical.cpp:24-49),
+// as sum_j a_j (f_j - f_c).
+template <int D, class T>
+__device__ __forceinline__ void lsq_grad(const KArgs &a, const T *__restrict__ f, int c, int v, T fc, T gr[3]) {
+    gr[0] = gr[1] = gr[2] = T(0);
+    const int j0 = a.nb_ptr[c], j1 = a.nb_ptr[c + 1];
+    const int nvp = a.s.nvp;
+    for (int j = j0; j < j1; ++j) {
+        T df = f[size_t(a.nb_ids[j]) * nvp + v] - fc;
+        if constexpr (sizeof(T) == 8) {
+            gr[0] += T(a.nb_a[3 * j]) * df;
+            gr[1] += T(a.nb_a[3 * j + 1]) * df;
+            if (D == 3) gr[2] += T(a.nb_a[3 * j + 2]) * df;
+        } else {
+            gr[0] += a.nb_af[3 * j] * df;
+            gr[1] += a.nb_af[3 * j + 1] * df;
+            if (D == 3) gr[2] += a.nb_af[3 * j + 2] * df;
+        }
+    }
+}
+
+template <class T>
+__device__ __forceinline__ T face_d(const KArgs &a, int f, int side, int k) {
+    if constexpr (sizeof(T) == 8) return a.face_d[6 * f + 3 * side + k];
+    else return a.face_df[6 * f + 3 * side + k];
+}
+
+template <class T>
+__device__ __forceinline__ void load_xi(const KArgs &a, int v, T x[3]) {
+    if constexpr (sizeof(T) == 8) {
+        x[0] = a.xi[3 * v];
+        x[1] = a.xi[3 * v + 1];
+        x[2] = a.xi[3 * v + 2];
+    } else {
+        x[0] = a.xif[3 * v];
+        x[1] = a.xif[3 * v + 1];
+        x[2] = a.xif[3 * v + 2];
+    }
+}
+
+// n0 . xi in FP64 with meso's evaluation order (x*x' + y*y' + z*z'); its sign
+// against +-VSMALL picks the upwind side.
+__device__ __forceinline__ double ndot(const KArgs &a, int f, int v) {
+    return a.face_n[3 * f] * a.xi[3 * v] + a.face_n[3 * f + 1] * a.xi[3 * v + 1] +
+           a.face_n[3 * f + 2] * a.xi[3 * v + 2];
+}
+
+// (d - xi h) . grad
+template <int D, class T>
+__device__ __forceinline__ T extrap(const KArgs &a, int f, int side, const T xh[3], const T gr[3]) {
+    T r = (face_d<T>(a, f, side, 0) - xh[0]) * gr[0] + (face_d<T>(a, f, side, 1) - xh[1]) * gr[1];
+    if (D == 3) r += (face_d<T>(a, f, side, 2) - xh[2]) * gr[2];
+    return r;
+}
+
+// f-bar+ of one distribution at face f, velocity v (cdugks_shakhov.cpp:223-298).
+template <int D, class T>
+__device__ __forceinline__ T fbar_at(const KArgs &a, const T *__restrict__ fd, int f, int own, int nei, double sn,
+                                     int v, const T xh[3]) {
+    const int nvp = a.s.nvp;
+    if (sn >= VSMALL || sn < -VSMALL) {
+        int up = (sn >= VSMALL) ? own : nei;
+        int side = (sn >= VSMALL) ? 0 : 1;
+        T fc = fd[size_t(up) * nvp + v];
+        if (!a.s.grad) return fc;
+        T gr[3];
+        lsq_grad<D, T>(a, fd, up, v, fc, gr);
+        return fc + extrap<D, T>(a, f, side, xh, gr);
+    }
+    T fo = fd[size_t(own) * nvp + v], fn = fd[size_t(nei) * nvp + v];
+    if (!a.s.grad) return T(0.5) * (fo + fn);
+    T go[3], gn[3];
+    lsq_grad<D, T>(a, fd, own, v, fo, go);
+    lsq_grad<D, T>(a, fd, nei, v, fn, gn);
+    return T(0.5) * (fo + fn + extrap<D, T>(a, f, 0, xh, go) + extrap<D, T>(a, f, 1, xh, gn));
+}
+
+}  // namespace meso_dev
