@@ -1281,9 +1281,11 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         if (mk.type == MESO_BC_FLUID_INTERIOR) continue;
         if (m->face_cells[2 * f] != m->face_cells[2 * f + 1])
             return fail(set_err(nullptr, MESO_E_INVALID, "boundary mark on an interior face"));
-        if ((mk.type == MESO_BC_SYMMETRY || mk.type == MESO_BC_SLIP_WALL) && !d->mirror &&
-            p->model == MESO_MODEL_CDUGKS_SHAKHOV)
-            return fail(set_err(nullptr, MESO_E_INVALID, "symmetry/slip-wall need DVS mirror ids"));
+        if ((mk.type == MESO_BC_SYMMETRY || mk.type == MESO_BC_SLIP_WALL) && p->model == MESO_MODEL_CDUGKS_SHAKHOV) {
+            if (!d->mirror) return fail(set_err(nullptr, MESO_E_INVALID, "symmetry/slip-wall need DVS mirror ids"));
+            if (mk.direction < 0 || mk.direction > 2)
+                return fail(set_err(nullptr, MESO_E_INVALID, "mirror direction must be 0, 1 or 2"));
+        }
         c->face_bnd_h[f] = int(c->bnd_face_h.size());
         c->bnd_face_h.push_back(f);
         bmark.push_back(m->face_mark[f]);

@@ -1,0 +1,164 @@
+"""GPU parity: the gfx950 step (through the C ABI) against the FP64 oracle on
+the same seeded inputs.
+
+Tolerances (relative, against the oracle):
+  FP64 storage: max-norm 1e-10 on rho, u, T, q and flux moments after 10-20
+  steps (summation order of the velocity reductions differs from meso's).
+  FP32 storage: relative L2 <= 2e-6 on every macro after 10 steps.
+"""
+import numpy as np
+import pytest
+
+from conftest import max_rel, rel_l2
+from meso_hip import capi
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 2e-6
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def run_pair(case_path, steps, storage=capi.STORAGE_FP64):
+    cs = capi.Case(case_path)
+    _say(f"[gpu] {case_path} NC={cs.n_cells} NV={cs.n_vel} model={cs.model} storage={storage} steps={steps}")
+    dev = capi.Device(cs, storage)
+    orc = Oracle(cs)
+    dev.init_equilibrium()
+    orc.init_equilibrium()
+    out = []
+    done = 0
+    for s in steps:
+        dev.step(s - done)
+        orc.step(s - done)
+        done = s
+        out.append((s, dev.macros(), orc.macros(), dev.flux_moments(), orc.flux_moments()))
+    return cs, dev, orc, out
+
+
+def keys_of(cs):
+    return ["rho", "u", "T", "q"] if cs.model == capi.MODEL_CDUGKS_SHAKHOV else ["rho", "u"]
+
+
+@pytest.mark.parametrize("name", ["bgk_quad8", "bgk_tri6", "dugks_quad8", "shak_quad6", "shak_tri6",
+                                  "shak_hex3", "shak_channel_pressure", "shak_channel_farfield",
+                                  "bgk_channel_farfield", "shak_sym_slip"])
+def test_fp64_matches_oracle(case_factory, name):
+    cs, dev, orc, out = run_pair(case_factory(name), [1, 2, 10])
+    for s, md, mo, fd, fo in out:
+        for k in keys_of(cs):
+            err = max_rel(md[k], mo[k])
+            assert err < FP64_TOL, f"{name} step {s} {k}: {err:.2e}"
+        if cs.model != capi.MODEL_DUGKS_BGK:
+            for a, b in zip(fd, fo):
+                # flux moments vanish at rest: absolute floor of 1e-14
+                assert np.allclose(a, b, rtol=1e-9, atol=1e-14), f"{name} step {s} flux moments"
+
+
+@pytest.mark.parametrize("name", ["bgk_quad8", "shak_quad6", "shak_tri6", "shak_hex3", "shak_quad12_nc24"])
+def test_fp32_storage_close_to_oracle(case_factory, name):
+    cs, dev, orc, out = run_pair(case_factory(name), [10], storage=capi.STORAGE_FP32)
+    s, md, mo, _, _ = out[-1]
+    for k in keys_of(cs):
+        if k == "q" or k == "u":
+            comps = [0, 1, 2] if cs.mesh.dim == 3 else [0, 1]
+            for i in comps:
+                err = rel_l2(md[k][:, i], mo[k][:, i])
+                assert err < FP32_TOL, f"{name} {k}[{i}]: {err:.2e}"
+        else:
+            err = rel_l2(md[k], mo[k])
+            assert err < FP32_TOL, f"{name} {k}: {err:.2e}"
+
+
+def test_state_roundtrip_and_restart(case_factory):
+    cs = capi.Case(case_factory("shak_quad6"))
+    dev = capi.Device(cs)
+    orc = Oracle(cs)
+    orc.init_equilibrium()
+    orc.step(3)
+    f = orc.get_state()
+    m = orc.macros()
+    dev.set_state(f, m["rho"], m["u"], m["T"], m["q"])
+    back = dev.get_state()
+    assert np.array_equal(back, f)
+    fc = dev.get_state(capi.LAYOUT_CELL_MAJOR)
+    assert np.array_equal(fc, np.transpose(f, (0, 2, 1)))
+    dev.step(4)
+    orc.step(4)
+    md, mo = dev.macros(), orc.macros()
+    for k in ["rho", "u", "T", "q"]:
+        assert max_rel(md[k], mo[k]) < FP64_TOL
+
+
+def test_init_from_macros_matches_oracle(case_factory):
+    cs = capi.Case(case_factory("shak_tri6"))
+    rng = np.random.default_rng(0)
+    nc = cs.n_cells
+    rho = 1.0 + 0.01 * rng.standard_normal(nc)
+    u = 0.05 * rng.standard_normal((nc, 3))
+    u[:, 2] = 0.0
+    T = 1.0 + 0.01 * rng.standard_normal(nc)
+    q = 0.01 * rng.standard_normal((nc, 3))
+    q[:, 2] = 0.0
+    dev, orc = capi.Device(cs), Oracle(cs)
+    dev.init_from_macros(rho, u, T, q)
+    orc.init_from_macros(rho, u, T, q)
+    assert max_rel(dev.get_state(), orc.get_state()) < 1e-14
+    dev.step(5)
+    orc.step(5)
+    md, mo = dev.macros(), orc.macros()
+    for k in ["rho", "u", "T", "q"]:
+        assert max_rel(md[k], mo[k]) < FP64_TOL
+
+
+def test_residual_matches_oracle(case_factory):
+    """Both sides start from the same developed state, so the stored previous
+    copies agree bit for bit (a rest state has u == 0 and inf residuals)."""
+    cs = capi.Case(case_factory("shak_quad6"))
+    dev, orc = capi.Device(cs), Oracle(cs)
+    orc.init_equilibrium()
+    orc.step(20)
+    m = orc.macros()
+    dev.set_state(orc.get_state(), m["rho"], m["u"], m["T"], m["q"])
+    orc.set_state(orc.get_state(), m["rho"], m["u"], m["T"], m["q"])
+    dev.step(5)
+    orc.step(5)
+    rd, ro = dev.residual(), orc.residual()
+    finite = np.isfinite(ro)
+    assert np.array_equal(np.isfinite(rd), finite)
+    assert np.allclose(rd[finite], ro[finite], rtol=1e-9, atol=0)
+
+
+def test_live_param_change(case_factory):
+    """dt / mu0 edits between steps (update_config() re-reads them every step)."""
+    cs = capi.Case(case_factory("shak_quad6"))
+    dev, orc = capi.Device(cs), Oracle(cs)
+    dev.init_equilibrium()
+    orc.init_equilibrium()
+    dev.step(2)
+    orc.step(2)
+    p = capi.StepParams.from_buffer_copy(cs.params)
+    p.dt *= 0.5
+    p.mu0 *= 1.5
+    dev.set_params(p)
+    orc.set_params(p)
+    dev.step(3)
+    orc.step(3)
+    md, mo = dev.macros(), orc.macros()
+    for k in ["rho", "u", "T", "q"]:
+        assert max_rel(md[k], mo[k]) < FP64_TOL
+
+
+def test_mass_conserved_in_closed_cavity(case_factory):
+    cs = capi.Case(case_factory("shak_quad12_nc24"))
+    dev = capi.Device(cs)
+    dev.init_equilibrium()
+    vol = cs.arrays()["cell_vol"]
+    m0 = float((dev.macros()["rho"] * vol).sum())
+    dev.step(50)
+    m1 = float((dev.macros()["rho"] * vol).sum())
+    assert abs(m1 - m0) / m0 < 1e-12
