@@ -1,0 +1,183 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 Shakhov CDUGKS step on the Kn = 1 cavity.
+
+Workload (one GPU): 200x200 quads jittered by 10% (40,000 cells), 101x101
+Newton-Cotes velocities on [-5, 5]^2 (10,201), Shakhov model, FP32 storage of
+g, h with FP64 moments/macros. A step = one full time step (all kernels) of
+the whole mesh; value = DV-cell updates/s = NC * NV * steps / seconds, inputs
+resident in HBM, timed with a device synchronisation on both sides.
+
+Also reported: the FP64-storage step of the same case (ratio FP32/FP64), the
+FP32-vs-FP64 relative L2 of every macro after the run, per-kernel times from
+HIP events, the roofline of the dominant kernel (16 B per DV-cell update:
+read + write of g and h in FP32) and the CPU baseline (the FP64 oracle with
+OpenMP on this host, one step of the same workload).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scale s]
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "meso-lts_amd"))
+
+from meso_hip import capi, configs  # noqa: E402
+
+METRIC = "DV-cell updates/s, HBM GB/s % of peak; FP32-store vs FP64 at <=1e-6 rel err"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "cavity_kn1_quad"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a - b))
+
+
+def time_steps(dev, steps, warmup):
+    dev.step(warmup)
+    dev.sync()
+    t0 = time.perf_counter()
+    rep = dev.step(steps)
+    dev.sync()
+    t1 = time.perf_counter()
+    if rep.bad_cells:
+        raise RuntimeError(f"numerical failure in the timed steps (cell {rep.first_bad_cell})")
+    return t1 - t0
+
+
+def kernel_profile(dev, steps):
+    dev.profile(True)
+    dev.step(steps)
+    kt = dev.kernel_times()
+    dev.profile(False)
+    return {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in kt.items()}
+
+
+def cpu_baseline(case_path, steps):
+    sys.path.insert(0, ROOT)
+    from oracle.oracle import Oracle, get_threads
+
+    cs = capi.Case(case_path)
+    orc = Oracle(cs)
+    orc.init_equilibrium()
+    t0 = time.perf_counter()
+    orc.step(steps)
+    dt = time.perf_counter() - t0
+    ups = cs.n_cells * cs.n_vel * steps / dt
+    return {"value": ups, "unit": "DV-cell updates/s", "cores": get_threads(), "kind": "port",
+            "sample": f"{steps} full step(s) of the same workload ({cs.n_cells} cells x {cs.n_vel} velocities), "
+                      f"FP64 oracle with OpenMP, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0, help="mesh resolution scale (<1: quick runs)")
+    ap.add_argument("--workload", default=WORKLOAD)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the step is benchmarked on one MI355X (--gpus {args.gpus} "
+              "is not supported; the multi-GPU velocity split is a design, not a measured path).", file=sys.stderr)
+        sys.exit(2)
+
+    work = os.path.join(tempfile.gettempdir(), "meso_hip_bench")
+    case_path = configs.build(args.workload, work, args.scale)
+    cs = capi.Case(case_path)
+    nc, nv = cs.n_cells, cs.n_vel
+    shakhov = cs.model == capi.MODEL_CDUGKS_SHAKHOV
+    nd = 2 if shakhov else 1
+    log(f"[bench] {args.workload}: NC={nc} NF={cs.n_faces} NV={nv} model={cs.model}")
+
+    res = {}
+    macros = {}
+    for storage, tag in ((capi.STORAGE_FP32, "fp32"), (capi.STORAGE_FP64, "fp64")):
+        dev = capi.Device(cs, storage)
+        dev.init_equilibrium()
+        secs = time_steps(dev, args.steps, args.warmup)
+        kt = kernel_profile(dev, min(args.steps, 5))
+        macros[tag] = dev.macros()
+        res[tag] = dict(seconds=secs, ups=nc * nv * args.steps / secs, kernels=kt, bytes=dev.device_bytes())
+        log(f"[bench] {tag}: {secs / args.steps * 1e3:.3f} ms/step, {res[tag]['ups']:.4e} DV-cell updates/s")
+        for k, v in sorted(kt.items(), key=lambda x: -x[1]["avg_ms"]):
+            log(f"          {k:14s} {v['avg_ms']:.4f} ms x{v['launches']}")
+        dev.close()
+
+    # precision of the FP32-storage step against the FP64 step (same steps)
+    prec = {}
+    a, b = macros["fp32"], macros["fp64"]
+    prec["rho"] = rel_l2(a["rho"], b["rho"])
+    dim = cs.mesh.dim
+    for i, c in enumerate("UVW"[:dim]):
+        prec[c] = rel_l2(a["u"][:, i], b["u"][:, i])
+    if shakhov:
+        prec["T"] = rel_l2(a["T"], b["T"])
+        for i, c in enumerate("xyz"[:dim]):
+            prec["q" + c] = rel_l2(a["q"][:, i], b["q"][:, i])
+
+    s = 4  # FP32 bytes
+    b_alg = 2 * nd * s
+    k32 = res["fp32"]["kernels"]
+    dom = max(k32, key=lambda k: k32[k]["avg_ms"])
+    dom_ms = k32[dom]["avg_ms"]
+    units = nc * nv  # DV-cells one launch of the cell kernel updates
+    achieved = b_alg * units / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fp:
+            traffic = json.load(fp).get(dom)
+    ms_step = res["fp32"]["seconds"] / args.steps * 1e3
+    step_gbs = b_alg * units / (ms_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(case_path, args.cpu_steps)
+            log(f"[bench] cpu baseline: {cpu['value']:.4e} DV-cell updates/s on {cpu['cores']} threads")
+        except Exception as e:  # the baseline is reported, never required
+            log(f"[bench] cpu baseline failed: {e}")
+
+    line = {
+        "metric": METRIC,
+        "value": res["fp32"]["ups"],
+        "unit": "DV-cell updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "cells": nc, "faces": cs.n_faces, "velocities": nv,
+                   "distributions": nd, "storage": "fp32 g,h; fp64 moments/macros", "mesh_scale": args.scale},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": b_alg * units, "avg_launch_ms": dom_ms},
+        "step_hbm": {"alg_bytes_per_step": b_alg * units, "achieved_GBs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
+        "fp64": {"ms_per_step": res["fp64"]["seconds"] / args.steps * 1e3, "value": res["fp64"]["ups"]},
+        "fp32_over_fp64": res["fp32"]["ups"] / res["fp64"]["ups"],
+        "fp32_vs_fp64_rel_l2": prec,
+        "kernels_ms": {k: v["avg_ms"] for k, v in k32.items()},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

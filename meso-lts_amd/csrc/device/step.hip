@@ -69,7 +69,7 @@ __device__ __forceinline__ T bnd_mix(const KArgs &a, const T *__restrict__ fd, i
         T X = extrap<D, T>(a, f, 0, xh, gr);
         fb = (sn >= VSMALL || sn < -VSMALL) ? fc + X : T(0.5) * (fc + fc + X + X);
     }
-    return (T(1) - mix) * fb + mix * eqv;
+    return lerp_store<T>(fb, eqv, mix);
 }
 
 // Face-mixed pair (g, h) of boundary face f at velocity v.
@@ -81,15 +81,10 @@ __device__ __forceinline__ void bnd_mix_pair(const KArgs &a, int f, int c, int v
     T xh[3] = {x[0] * hd, x[1] * hd, x[2] * hd};
     double sn = ndot(a, f, v);
     T mix = T(ef.mix);
-    if (MODEL == CDUGKS_SHAKHOV) {
-        T gs, hs;
-        shakhov_eq<T>(ef, x, D, a.s.K, gs, hs);
-        g = bnd_mix<D, T>(a, (const T *)a.fin_g, f, c, sn, v, xh, gs, mix);
-        h = bnd_mix<D, T>(a, (const T *)a.fin_h, f, c, sn, v, xh, hs, mix);
-    } else {
-        g = bnd_mix<D, T>(a, (const T *)a.fin_g, f, c, sn, v, xh, bgk_eq<T>(ef, x), mix);
-        h = T(0);
-    }
+    T gs, hs;
+    eq_store<MODEL, D, T>(a, ef, x, v, gs, hs);
+    g = bnd_mix<D, T>(a, (const T *)a.fin_g, f, c, sn, v, xh, gs, mix);
+    h = MODEL == CDUGKS_SHAKHOV ? bnd_mix<D, T>(a, (const T *)a.fin_h, f, c, sn, v, xh, hs, mix) : T(0);
 }
 
 // Final boundary value at velocity v given the mixed value (gm_, hm_) there.
@@ -103,10 +98,7 @@ __device__ __forceinline__ void bnd_value(const KArgs &a, const BndParams &P, in
     T x[3];
     load_xi<T>(a, v, x);
     T gi = T(0), hi = T(0);
-    if (P.kind != BC_SYMMETRY) {
-        if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<T>(P.in, x, D, a.s.K, gi, hi);
-        else gi = bgk_eq<T>(P.in, x);
-    }
+    if (P.kind != BC_SYMMETRY) eq_store<MODEL, D, T>(a, P.in, x, v, gi, hi);
     switch (P.kind) {
         case BC_FARFIELD_INLET:
         case BC_PRESSURE_INLET:
@@ -252,6 +244,10 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
         for (int t = 0; t < a.s.ntile_f; ++t)
 #pragma unroll
             for (int k = 0; k < NCOL; ++k) M[k] += src[t * NCOL + k];
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int k = 0; k < NCOL; ++k) M[k] += a.mref[k];
+        }
         const double n[3] = {a.face_n[3 * f], a.face_n[3 * f + 1], a.face_n[3 * f + 2]};
         const double rho = M[Col<D>::M0];
         const double inv = 1.0 / rho;
@@ -292,6 +288,7 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
                 q[i] = cq * m3;
             }
             make_shakhov(e, rho, u, T_, q, a.s.R, a.s.Pr, D, a.s.K);
+            if constexpr (sizeof(T) == 4) relate_shakhov(e, a.ref);
             e.mix = hdt / (2.0 * tau + hdt);
             double M3n = 0.0, H1n = 0.0;
             for (int i = 0; i < D; ++i) {
@@ -301,6 +298,7 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
             Fb[D + 1] = M3n + H1n;
         } else {
             make_bgk(e, rho, u, a.s.RT);
+            if constexpr (sizeof(T) == 4) relate_bgk(e, a.ref);
             e.mix = hdt / (2.0 * a.s.tau + hdt);
         }
         if (li == 0) a.face_eq[f] = e;
@@ -312,22 +310,14 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
                 load_xi<T>(a, v, x);
                 const double X0 = a.xi[3 * v], X1 = a.xi[3 * v + 1], X2 = a.xi[3 * v + 2];
                 const double wsn = a.w[v] * (n[0] * X0 + n[1] * X1 + n[2] * X2);
-                if (MODEL == CDUGKS_SHAKHOV) {
-                    T gs, hs;
-                    shakhov_eq<T>(e, x, D, a.s.K, gs, hs);
-                    const double wg = wsn * double(gs);
-                    acc[0] += wg;
-                    acc[1] += wg * X0;
-                    acc[2] += wg * X1;
-                    if (D == 3) acc[3] += wg * X2;
-                    acc[D + 1] += wg * (X0 * X0 + X1 * X1 + X2 * X2) + wsn * double(hs);
-                } else {
-                    const double wg = wsn * double(bgk_eq<T>(e, x));
-                    acc[0] += wg;
-                    acc[1] += wg * X0;
-                    acc[2] += wg * X1;
-                    if (D == 3) acc[3] += wg * X2;
-                }
+                T gs, hs;
+                eq_store<MODEL, D, T>(a, e, x, v, gs, hs);
+                const double wg = wsn * double(gs);
+                acc[0] += wg;
+                acc[1] += wg * X0;
+                acc[2] += wg * X1;
+                if (D == 3) acc[3] += wg * X2;
+                if (MODEL == CDUGKS_SHAKHOV) acc[D + 1] += wg * (X0 * X0 + X1 * X1 + X2 * X2) + wsn * double(hs);
             }
         }
     }
@@ -335,6 +325,11 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
 #pragma unroll
     for (int k = 0; k < NFL; ++k) acc[k] = seg_sum<LP>(acc[k]);
     if (act && interior && li == 0) {
+        if constexpr (sizeof(T) == 4) {  // flux moments of f_ref along n
+            const double n[3] = {a.face_n[3 * f], a.face_n[3 * f + 1], a.face_n[3 * f + 2]};
+            for (int k = 0; k < NFL; ++k)
+                acc[k] += n[0] * a.rflux[k][0] + n[1] * a.rflux[k][1] + n[2] * a.rflux[k][2];
+        }
         const double mix = e.mix;
         const int nfl = MODEL == CDUGKS_SHAKHOV ? D + 2 : D + 1;
         for (int k = 0; k < nfl; ++k) a.face_flux[size_t(f) * NFL + k] = (1.0 - mix) * Fb[k] + mix * acc[k];
@@ -393,6 +388,7 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
     const double R = a.s.R, Pr = a.s.Pr;
     const int K = a.s.K;
     make_bgk(P.in, 1.0, zero, 1.0);
+    P.in.A = 0.0;
     switch (P.kind) {
         case BC_FARFIELD_INLET:
             if (MODEL == CDUGKS_SHAKHOV) make_shakhov(P.in, mk.density, uw, mk.temperature, zero, R, Pr, D, K);
@@ -428,16 +424,15 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
             for (int v = threadIdx.x; v < nv; v += BLOCK) {
                 const double kn = -ndot(a, f, v);
                 if (kn >= VSMALL) {
-                    T x[3];
-                    load_xi<T>(a, v, x);
-                    T gm, hm;
-                    if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<T>(e1, x, D, K, gm, hm);
-                    else gm = bgk_eq<T>(e1, x);
-                    s[1] += kn * a.w[v] * double(gm);
+                    const double X[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
+                    double gm, hm;
+                    if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<double>(e1, X, D, K, gm, hm);
+                    else gm = bgk_eq<double>(e1, X);
+                    s[1] += kn * a.w[v] * gm;
                 } else {
                     T gmx, hmx;
                     bnd_mix_pair<MODEL, D, T>(a, f, c, v, ef, gmx, hmx);
-                    s[0] -= kn * a.w[v] * double(gmx);
+                    s[0] -= kn * a.w[v] * full_value<T>(a, gmx, 0, v);
                 }
             }
             block_sum<2>(s, lds);
@@ -446,6 +441,12 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
             else make_bgk(P.in, rho_w, uw, a.s.RT);
         } break;
         default: break;
+    }
+    if constexpr (sizeof(T) == 4) {
+        if (P.kind != BC_SYMMETRY && P.kind != BC_INTERIOR) {
+            if (MODEL == CDUGKS_SHAKHOV) relate_shakhov(P.in, a.ref);
+            else relate_bgk(P.in, a.ref);
+        }
     }
     double acc[NFL];
 #pragma unroll
@@ -457,12 +458,13 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
         bnd_value<MODEL, D, T>(a, P, f, c, v, sn, ef, gmx, hmx, g, h);
         const double X0 = a.xi[3 * v], X1 = a.xi[3 * v + 1], X2 = a.xi[3 * v + 2];
         const double wsn = a.w[v] * sn;
-        const double wg = wsn * double(g);
+        const double wg = wsn * full_value<T>(a, g, 0, v);
         acc[0] += wg;
         acc[1] += wg * X0;
         acc[2] += wg * X1;
         if (D == 3) acc[3] += wg * X2;
-        if (MODEL == CDUGKS_SHAKHOV) acc[D + 1] += wg * (X0 * X0 + X1 * X1 + X2 * X2) + wsn * double(h);
+        if (MODEL == CDUGKS_SHAKHOV)
+            acc[D + 1] += wg * (X0 * X0 + X1 * X1 + X2 * X2) + wsn * full_value<T>(a, h, 1, v);
     }
     block_sum<NFL>(acc, lds);
     if (threadIdx.x == 0) {
@@ -481,7 +483,7 @@ struct CellStep {
     double phi[D + 2];
 };
 
-template <int MODEL, int D>
+template <int MODEL, int D, class T>
 __device__ __forceinline__ bool cell_macros(const KArgs &a, int c, CellStep<MODEL, D> &cs) {
     constexpr int NFL = D + 2;
     for (int k = 0; k < NFL; ++k) cs.phi[k] = 0.0;
@@ -517,6 +519,10 @@ __device__ __forceinline__ bool cell_macros(const KArgs &a, int c, CellStep<MODE
         cs.Cs = hdt / (2.0 * cs.tau);
         make_shakhov(cs.en, rn, un, Tn, qn, a.s.R, a.s.Pr, D, a.s.K);
         make_shakhov(cs.e, r, cs.u, cs.T, qn, a.s.R, a.s.Pr, D, a.s.K);
+        if constexpr (sizeof(T) == 4) {
+            relate_shakhov(cs.en, a.ref);
+            relate_shakhov(cs.e, a.ref);
+        }
         return !(isnan(r) || cs.T < 0.0 || isnan(cs.T));
     } else if (MODEL == CDUGKS_BGK) {
         const double tau = a.s.tau;
@@ -531,6 +537,10 @@ __device__ __forceinline__ bool cell_macros(const KArgs &a, int c, CellStep<MODE
         cs.Cs = hdt / (2.0 * tau);
         make_bgk(cs.en, rn, un, a.s.RT);
         make_bgk(cs.e, r, cs.u, a.s.RT);
+        if constexpr (sizeof(T) == 4) {
+            relate_bgk(cs.en, a.ref);
+            relate_bgk(cs.e, a.ref);
+        }
         return !isnan(r);
     } else {  // classic DUGKS: macros follow from f-tilde^{n+1} (dugks.cpp:313-335)
         cs.rho = rn;
@@ -539,6 +549,7 @@ __device__ __forceinline__ bool cell_macros(const KArgs &a, int c, CellStep<MODE
         cs.tau = cs.taun = a.s.tau;
         cs.cm = cs.C = cs.Cs = 0.0;
         make_bgk(cs.en, rn, un, a.s.RT);
+        if constexpr (sizeof(T) == 4) relate_bgk(cs.en, a.ref);
         cs.e = cs.en;
         return true;
     }
@@ -586,7 +597,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a) {
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     if (act) {
         CellStep<MODEL, D> cs;
-        const bool ok = cell_macros<MODEL, D>(a, c, cs);
+        const bool ok = cell_macros<MODEL, D, T>(a, c, cs);
         if (tile == 0 && li == 0) {
             if (MODEL != DUGKS_BGK) {
                 a.rho_out[c] = cs.rho;
@@ -644,52 +655,88 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a) {
                         hv = hmx;
                     }
                 } else {
-                    if (MODEL == CDUGKS_SHAKHOV) {
-                        T gs, hs;
-                        shakhov_eq<T>(ef, x, D, a.s.K, gs, hs);
-                        gv = (T(1) - mix) * interior_fbar<D, T>(a, fg, f, oc, c, sn, v, xh, fcg, gcg) + mix * gs;
-                        hv = (T(1) - mix) * interior_fbar<D, T>(a, fh, f, oc, c, sn, v, xh, fch, gch) + mix * hs;
-                    } else {
-                        gv = (T(1) - mix) * interior_fbar<D, T>(a, fg, f, oc, c, sn, v, xh, fcg, gcg) +
-                             mix * bgk_eq<T>(ef, x);
-                    }
+                    T gs, hs;
+                    eq_store<MODEL, D, T>(a, ef, x, v, gs, hs);
+                    gv = lerp_store<T>(interior_fbar<D, T>(a, fg, f, oc, c, sn, v, xh, fcg, gcg), gs, mix);
+                    if (MODEL == CDUGKS_SHAKHOV)
+                        hv = lerp_store<T>(interior_fbar<D, T>(a, fh, f, oc, c, sn, v, xh, fch, gch), hs, mix);
                 }
                 const T knA = T(sn) * T(a.cf_sA[j]);
                 Fg += knA * gv;
                 if (MODEL == CDUGKS_SHAKHOV) Fh += knA * hv;
             }
+            if constexpr (sizeof(T) == 4) {
+                // flux of f_ref itself: f_ref xi . sum(+-A n); non-zero where faces are not planar
+                const float xs = x[0] * a.cell_S[3 * c] + x[1] * a.cell_S[3 * c + 1] + x[2] * a.cell_S[3 * c + 2];
+                Fg += a.fref_f[v] * xs;
+                if (MODEL == CDUGKS_SHAKHOV) Fh += a.fref_f[a.s.nv + v] * xs;
+            }
             const double X0 = a.xi[3 * v], X1 = a.xi[3 * v + 1], X2 = a.xi[3 * v + 2];
             const double wv = a.w[v];
             if (MODEL == CDUGKS_SHAKHOV) {
                 T gsn, hsn, gs, hs;
-                shakhov_eq<T>(cs.en, x, D, a.s.K, gsn, hsn);
-                shakhov_eq<T>(cs.e, x, D, a.s.K, gs, hs);
+                eq_store<MODEL, D, T>(a, cs.en, x, v, gsn, hsn);
+                eq_store<MODEL, D, T>(a, cs.e, x, v, gs, hs);
                 const T cm = T(cs.cm), C = T(cs.C), Cs = T(cs.Cs);
-                const T itau = T(1.0 / cs.tau), itaun = T(1.0 / cs.taun);
-                const T gn = cm * gsn + (T(1) - cm) * fcg;
-                const T hn = cm * hsn + (T(1) - cm) * fch;
-                const T g = C * (gn + hd * (gs * itau + (gsn - gn) * itaun) - dtv * Fg);
-                const T h = C * (hn + hd * (hs * itau + (hsn - hn) * itaun) - dtv * Fh);
+                T g, h, og_, oh_;
+                if constexpr (sizeof(T) == 8) {  // cdugks_shakhov.cpp:679-695 as written
+                    const T itau = T(1.0 / cs.tau), itaun = T(1.0 / cs.taun);
+                    const T gn = cm * gsn + (T(1) - cm) * fcg;
+                    const T hn = cm * hsn + (T(1) - cm) * fch;
+                    g = C * (gn + hd * (gs * itau + (gsn - gn) * itaun) - dtv * Fg);
+                    h = C * (hn + hd * (hs * itau + (hsn - hn) * itaun) - dtv * Fh);
+                    og_ = (T(1) - Cs) * g + Cs * gs;
+                    oh_ = (T(1) - Cs) * h + Cs * hs;
+                } else {
+                    // the same update as increments of the stored deviation:
+                    // g = g_n + C [a (g_S - g_n) + a_n (g_S^n - g_n) - dt/V F],
+                    // a = h/tau, a_n = h/tau_n, C = 1/(1 + a)
+                    const T ai = T(a.s.half_dt / cs.tau), an = T(a.s.half_dt / cs.taun);
+                    const T gn = fcg + cm * (gsn - fcg);
+                    const T hn = fch + cm * (hsn - fch);
+                    g = gn + C * (ai * (gs - gn) + an * (gsn - gn) - dtv * Fg);
+                    h = hn + C * (ai * (hs - hn) + an * (hsn - hn) - dtv * Fh);
+                    og_ = g + Cs * (gs - g);
+                    oh_ = h + Cs * (hs - h);
+                }
                 const double c0 = X0 - cs.u[0], c1 = X1 - cs.u[1], c2 = X2 - cs.u[2];
-                const double t = 0.5 * wv * ((c0 * c0 + c1 * c1 + c2 * c2) * double(g) + double(h));
+                const double t = 0.5 * wv * ((c0 * c0 + c1 * c1 + c2 * c2) * full_value<T>(a, g, 0, v) +
+                                             full_value<T>(a, h, 1, v));
                 acc[0] += t * c0;
                 acc[1] += t * c1;
                 acc[2] += t * c2;
-                og[row + v] = (T(1) - Cs) * g + Cs * gs;
-                oh[row + v] = (T(1) - Cs) * h + Cs * hs;
+                og[row + v] = og_;
+                oh[row + v] = oh_;
             } else if (MODEL == CDUGKS_BGK) {
-                const T feqn = bgk_eq<T>(cs.en, x), feq = bgk_eq<T>(cs.e, x);
-                const T cm = T(cs.cm), C = T(cs.C), ceq = T(cs.Cs), itau = T(1.0 / cs.tau);
-                const T fn = cm * feqn + (T(1) - cm) * fcg;
-                og[row + v] = (T(1) - ceq) * (C * (fn + hd * (feq * itau + (feqn - fn) * itau) - dtv * Fg)) + ceq * feq;
+                T feqn, feq, dummy;
+                eq_store<MODEL, D, T>(a, cs.en, x, v, feqn, dummy);
+                eq_store<MODEL, D, T>(a, cs.e, x, v, feq, dummy);
+                const T cm = T(cs.cm), C = T(cs.C), ceq = T(cs.Cs);
+                if constexpr (sizeof(T) == 8) {  // cdugks.cpp:325-332 as written
+                    const T itau = T(1.0 / cs.tau);
+                    const T fn = cm * feqn + (T(1) - cm) * fcg;
+                    og[row + v] =
+                        (T(1) - ceq) * (C * (fn + hd * (feq * itau + (feqn - fn) * itau) - dtv * Fg)) + ceq * feq;
+                } else {
+                    const T ai = T(a.s.half_dt / cs.tau);
+                    const T fn = fcg + cm * (feqn - fcg);
+                    const T g = fn + C * (ai * (feq - fn) + ai * (feqn - fn) - dtv * Fg);
+                    og[row + v] = g + ceq * (feq - g);
+                }
             } else {
                 const double dt = a.s.dt, tau = a.s.tau;
                 const T Cbt = T((dt + a.s.half_dt) / (a.s.half_dt - 2.0 * tau));
-                const T feq = bgk_eq<T>(cs.en, x);
-                const T ft = (T(1) - Cbt) * fcg + Cbt * feq;
-                const T fnew = (T(4) * fcg - ft) / T(3) - dtv * Fg;
+                T feq, dummy;
+                eq_store<MODEL, D, T>(a, cs.en, x, v, feq, dummy);
+                T fnew;
+                if constexpr (sizeof(T) == 8) {  // dugks.cpp:306-309 as written
+                    const T ft = (T(1) - Cbt) * fcg + Cbt * feq;
+                    fnew = (T(4) * fcg - ft) / T(3) - dtv * Fg;
+                } else {
+                    fnew = fcg - Cbt * (feq - fcg) / T(3) - dtv * Fg;
+                }
                 og[row + v] = fnew;
-                const double wf = wv * double(fnew);
+                const double wf = wv * full_value<T>(a, fnew, 0, v);
                 acc[0] += wf;
                 acc[1] += wf * X0;
                 acc[2] += wf * X1;
@@ -740,12 +787,15 @@ __global__ __launch_bounds__(BLOCK) void k_dugks_tobp(KArgs a) {
     EqParams e;
     const double u[3] = {a.u_out[3 * c], a.u_out[3 * c + 1], a.u_out[3 * c + 2]};
     make_bgk(e, a.rho_out[c], u, a.s.RT);
+    if constexpr (sizeof(T) == 4) relate_bgk(e, a.ref);
     const T C = T((a.s.dt + a.s.half_dt) / (2.0 * a.s.tau + a.s.dt));
     T *o = (T *)a.fout_g + size_t(c) * a.s.nvp;
     for (int v = li; v < a.s.nv; v += LP) {
         T x[3];
         load_xi<T>(a, v, x);
-        o[v] = (T(1) - C) * o[v] + C * bgk_eq<T>(e, x);
+        T feq, dummy;
+        eq_store<DUGKS_BGK, D, T>(a, e, x, v, feq, dummy);
+        o[v] = lerp_store<T>(o[v], feq, C);
     }
 }
 
@@ -771,11 +821,11 @@ __global__ __launch_bounds__(BLOCK) void k_init_eq(KArgs a, const double *rp, co
             double g, h = 0.0;
             if (MODEL == CDUGKS_SHAKHOV) {
                 shakhov_eq<double>(e, X, D, a.s.K, g, h);
-                oh[v] = T(h);
+                oh[v] = T(h - (sizeof(T) == 4 ? a.fref_d[a.s.nv + v] : 0.0));
             } else {
                 g = bgk_eq<double>(e, X);
             }
-            og[v] = T(g);
+            og[v] = T(g - (sizeof(T) == 4 ? a.fref_d[v] : 0.0));
             const double kk = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
             const double c0 = X[0] - u[0], c1 = X[1] - u[1], c2 = X[2] - u[2];
             const double cc = c0 * c0 + c1 * c1 + c2 * c2;
@@ -824,13 +874,14 @@ __global__ __launch_bounds__(BLOCK) void k_init_macros(KArgs a) {
     T *oh = (T *)a.fout_h + size_t(c) * a.s.nvp;
     for (int v = li; v < a.s.nv; v += LP) {
         const double X[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
+        const double rg = sizeof(T) == 4 ? a.fref_d[v] : 0.0;
         if (MODEL == CDUGKS_SHAKHOV) {
             double g, h;
             shakhov_eq<double>(e, X, D, a.s.K, g, h);
-            og[v] = T(g);
-            oh[v] = T(h);
+            og[v] = T(g - rg);
+            oh[v] = T(h - (sizeof(T) == 4 ? a.fref_d[a.s.nv + v] : 0.0));
         } else {
-            og[v] = T(bgk_eq<double>(e, X));
+            og[v] = T(bgk_eq<double>(e, X) - rg);
         }
     }
 }
@@ -838,7 +889,7 @@ __global__ __launch_bounds__(BLOCK) void k_init_macros(KArgs a) {
 // FP64 velocity-major [nv][nc] <-> device cell-major [nc][nvp] (32x32 tiles through LDS)
 template <class T, bool TO_DEVICE>
 __global__ __launch_bounds__(BLOCK) void k_transpose(const double *vm_in, double *vm_out, const T *cm_in, T *cm_out,
-                                                     int nc, int nv, int nvp) {
+                                                     int nc, int nv, int nvp, const double *fref) {
     __shared__ double tile[32][33];
     const int c0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
     const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
@@ -850,12 +901,12 @@ __global__ __launch_bounds__(BLOCK) void k_transpose(const double *vm_in, double
         __syncthreads();
         for (int k = ty; k < 32; k += 8) {
             int c = c0 + k, v = v0 + tx;
-            if (c < nc && v < nv) cm_out[size_t(c) * nvp + v] = T(tile[tx][k]);
+            if (c < nc && v < nv) cm_out[size_t(c) * nvp + v] = T(tile[tx][k] - (fref ? fref[v] : 0.0));
         }
     } else {
         for (int k = ty; k < 32; k += 8) {
             int c = c0 + k, v = v0 + tx;
-            tile[tx][k] = (c < nc && v < nv) ? double(cm_in[size_t(c) * nvp + v]) : 0.0;
+            tile[tx][k] = (c < nc && v < nv) ? double(cm_in[size_t(c) * nvp + v]) + (fref ? fref[v] : 0.0) : 0.0;
         }
         __syncthreads();
         for (int k = ty; k < 32; k += 8) {
@@ -921,7 +972,7 @@ struct meso_hip_ctx {
     // device pointers
     int2 *face_cells = nullptr;
     double *face_n = nullptr, *face_d = nullptr, *cf_sA = nullptr, *nb_a = nullptr, *cell_vol = nullptr;
-    float *face_df = nullptr, *nb_af = nullptr, *xif = nullptr;
+    float *face_df = nullptr, *nb_af = nullptr, *xif = nullptr, *cell_S = nullptr;
     int *cf_ptr = nullptr, *cf_ids = nullptr, *nb_ptr = nullptr, *nb_ids = nullptr, *face_bnd = nullptr,
         *bnd_face = nullptr, *bnd_mark = nullptr, *mirror = nullptr;
     meso_mark_desc *marks = nullptr;
@@ -935,6 +986,13 @@ struct meso_hip_ctx {
     EqParams *face_eq = nullptr;
     BndParams *bnd = nullptr;
     int *flags = nullptr;  // bad_count, first_bad, bad_step, halt
+    // reference equilibrium of the FP32 deviation storage
+    EqParams ref{};
+    double *fref_d = nullptr;
+    float *fref_f = nullptr;
+    double mref[17] = {};
+    double rflux[5][3] = {};
+    std::vector<double> xi_h, w_h;
     int *flags_host = nullptr;
     int cur = 0;           // buffer holding f^n / macros^n
     int steps_done = 0;
@@ -1021,6 +1079,7 @@ KArgs make_args(meso_hip_ctx *c) {
     a.nb_a = c->nb_a;
     a.nb_af = c->nb_af;
     a.cell_vol = c->cell_vol;
+    a.cell_S = c->cell_S;
     a.face_bnd = c->face_bnd;
     a.bnd_face = c->bnd_face;
     a.bnd_mark = c->bnd_mark;
@@ -1052,7 +1111,79 @@ KArgs make_args(meso_hip_ctx *c) {
     a.first_bad = c->flags + 1;
     a.bad_step = c->flags + 2;
     a.halt = c->flags + 3;
+    a.ref = c->ref;
+    a.fref_d = c->fref_d;
+    a.fref_f = c->fref_f;
+    std::memcpy(a.mref, c->mref, sizeof(a.mref));
+    std::memcpy(a.rflux, c->rflux, sizeof(a.rflux));
     return a;
+}
+
+// Reference equilibrium f_ref of the FP32 deviation storage: the Shakhov
+// (Maxwellian) pair, or the BGK polynomial, of (rho_ref, u = 0, T_ref); its
+// discrete raw moments in the face-moment column layout; and its flux vectors
+// sum_v w xi psi_k f_ref. Storage then holds f - f_ref, whose rounding error is
+// proportional to the deviation from rest instead of to f.
+int set_reference(meso_hip_ctx *c, double rho_ref, double T_ref) {
+    const StepConst &s = c->s;
+    const int nv = s.nv, D = s.D;
+    const bool shak = s.model == MESO_MODEL_CDUGKS_SHAKHOV;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    if (shak) make_shakhov(c->ref, rho_ref, zero, T_ref, zero, s.R, s.Pr, D, s.K);
+    else make_bgk(c->ref, rho_ref, zero, s.RT);
+    std::vector<double> fr(2 * size_t(nv), 0.0);
+    std::vector<float> ff(2 * size_t(nv), 0.0f);
+    std::memset(c->mref, 0, sizeof(c->mref));
+    std::memset(c->rflux, 0, sizeof(c->rflux));
+    for (int v = 0; v < nv; ++v) {
+        const double *x = &c->xi_h[3 * v];
+        const double xx = x[0] * x[0] + x[1] * x[1] + x[2] * x[2];
+        double g, h = 0.0;
+        if (shak) {
+            g = c->ref.A * std::exp(-c->ref.B * xx);
+            h = c->ref.hk * g;
+        } else {
+            g = rho_ref;
+        }
+        fr[v] = g;
+        fr[nv + v] = h;
+        ff[v] = float(g);
+        ff[nv + v] = float(h);
+        const double wg = c->w_h[v] * g, wh = c->w_h[v] * h;
+        int k = 0;
+        c->mref[k++] += wg;
+        for (int i = 0; i < D; ++i) c->mref[k++] += wg * x[i];
+        for (int i = 0; i < D; ++i)
+            for (int j = i; j < D; ++j) c->mref[k++] += wg * x[i] * x[j];
+        if (shak) {
+            for (int i = 0; i < D; ++i) c->mref[k++] += wg * xx * x[i];
+            c->mref[k++] += wh;
+            for (int i = 0; i < D; ++i) c->mref[k++] += wh * x[i];
+        }
+        for (int d = 0; d < 3; ++d) {
+            c->rflux[0][d] += wg * x[d];
+            for (int i = 0; i < D; ++i) c->rflux[1 + i][d] += wg * x[d] * x[i];
+            if (shak) c->rflux[D + 1][d] += x[d] * (wg * xx + wh);
+        }
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->fref_d, fr.data(), fr.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->fref_f, ff.data(), ff.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    return MESO_OK;
+}
+
+int set_reference_from(meso_hip_ctx *c, const double *rho, const double *T) {
+    if (c->storage != MESO_STORAGE_FP32) return MESO_OK;
+    const int nc = c->s.nc;
+    double r = 0.0, t = 0.0;
+    for (int i = 0; i < nc; ++i) {
+        r += rho[i];
+        t += T ? T[i] : 0.0;
+    }
+    r /= nc;
+    t = T ? t / nc : 1.0;
+    if (!(r > 0.0) || !(t > 0.0)) return set_err(c, MESO_E_INVALID, "reference state needs positive density and T");
+    return set_reference(c, r, t);
 }
 
 struct Launch {
@@ -1241,7 +1372,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             return fail(set_err(nullptr, MESO_E_INVALID, "neighbour index out of range"));
     std::vector<int2> fc(nf);
     std::vector<double> fd(6 * size_t(nf)), sA(m->n_cell_faces), na(3 * size_t(m->n_nbr));
-    std::vector<float> fdf(6 * size_t(nf)), naf(3 * size_t(m->n_nbr)), xif(3 * size_t(nv));
+    std::vector<float> fdf(6 * size_t(nf)), naf(3 * size_t(m->n_nbr)), xif(3 * size_t(nv)), cS(3 * size_t(nc), 0.f);
     for (int f = 0; f < nf; ++f) {
         fc[f] = make_int2(m->face_cells[2 * f], m->face_cells[2 * f + 1]);
         for (int side = 0; side < 2; ++side) {
@@ -1258,6 +1389,10 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             int f = m->cell_face_ids[k];
             sA[k] = (m->face_cells[2 * f] == cc) ? m->face_area[f] : -m->face_area[f];
         }
+        double S3[3] = {0.0, 0.0, 0.0};
+        for (int k = m->cell_face_ptr[cc]; k < m->cell_face_ptr[cc + 1]; ++k)
+            for (int r = 0; r < 3; ++r) S3[r] += sA[k] * m->face_normal[3 * size_t(m->cell_face_ids[k]) + r];
+        for (int r = 0; r < 3; ++r) cS[3 * size_t(cc) + r] = float(S3[r]);
         const double *C = m->lsq_C + 9 * size_t(cc);
         for (int j = m->nbr_ptr[cc]; j < m->nbr_ptr[cc + 1]; ++j) {
             const double *dr = m->lsq_dr + 3 * size_t(j);
@@ -1310,6 +1445,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     UP(nb_a, na.data(), na.size());
     UP(nb_af, naf.data(), naf.size());
     UP(cell_vol, m->cell_vol, nc);
+    UP(cell_S, cS.data(), cS.size());
     UP(face_bnd, c->face_bnd_h.data(), nf);
     UP(bnd_face, c->bnd_face_h.data(), c->bnd_face_h.size());
     UP(bnd_mark, bmark.data(), bmark.size());
@@ -1340,8 +1476,11 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         (rc = dalloc(c, &c->part_c, size_t(nc) * c->s.ntile_c * 4)) || (rc = dalloc(c, &c->fm, 5 * size_t(nc))) ||
         (rc = dalloc(c, &c->res_part, 8 * size_t((nc + BLOCK - 1) / BLOCK))) || (rc = dalloc(c, &c->res_out, 8)) ||
         (rc = dalloc(c, &c->face_eq, nf)) || (rc = dalloc(c, &c->bnd, std::max(1, c->s.nb))) ||
-        (rc = dalloc(c, &c->flags, 4)))
+        (rc = dalloc(c, &c->flags, 4)) || (rc = dalloc(c, &c->fref_d, 2 * size_t(nv))) ||
+        (rc = dalloc(c, &c->fref_f, 2 * size_t(nv))))
         return fail(rc);
+    c->xi_h.assign(d->xi, d->xi + 3 * size_t(nv));
+    c->w_h.assign(d->w, d->w + nv);
     if (hipHostMalloc((void **)&c->flags_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(set_err(nullptr, MESO_E_DEVICE, "hipHostMalloc failed"));
     if ((rc = reset_flags(c))) return fail(rc);
@@ -1378,6 +1517,7 @@ static int init_common(meso_hip_ctx *c, const double *rho, const double *u3, con
     const size_t nc = c->s.nc;
     double *tr = nullptr, *tu = nullptr, *tT = nullptr;
     int rc;
+    if ((rc = set_reference_from(c, rho, shak ? T : nullptr))) return rc;
     // the init kernels write buffer `1 - cur`, then it becomes current
     KArgs a = make_args(c);
     if (from_macros) {
@@ -1449,23 +1589,24 @@ static int transpose(meso_hip_ctx *c, bool to_device, double *host, int which, i
     dim3 grid((nc + 31) / 32, (nv + 31) / 32);
     void *dev = c->f[c->cur][which];
     const bool f32 = c->storage == MESO_STORAGE_FP32;
+    const double *fref = f32 ? c->fref_d + size_t(which) * nv : nullptr;
     if (to_device) {
         HIP_TRY(c, hipMemcpyAsync(stage, src, n * 8, hipMemcpyHostToDevice, c->stream));
         if (f32)
             hipLaunchKernelGGL((k_transpose<float, true>), grid, dim3(BLOCK), 0, c->stream, stage, nullptr, nullptr,
-                               (float *)dev, nc, nv, nvp);
+                               (float *)dev, nc, nv, nvp, fref);
         else
             hipLaunchKernelGGL((k_transpose<double, true>), grid, dim3(BLOCK), 0, c->stream, stage, nullptr, nullptr,
-                               (double *)dev, nc, nv, nvp);
+                               (double *)dev, nc, nv, nvp, fref);
         HIP_TRY(c, hipGetLastError());
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     } else {
         if (f32)
             hipLaunchKernelGGL((k_transpose<float, false>), grid, dim3(BLOCK), 0, c->stream, nullptr, stage,
-                               (const float *)dev, nullptr, nc, nv, nvp);
+                               (const float *)dev, nullptr, nc, nv, nvp, fref);
         else
             hipLaunchKernelGGL((k_transpose<double, false>), grid, dim3(BLOCK), 0, c->stream, nullptr, stage,
-                               (const double *)dev, nullptr, nc, nv, nvp);
+                               (const double *)dev, nullptr, nc, nv, nvp, fref);
         HIP_TRY(c, hipGetLastError());
         double *dst = host + size_t(which) * n;
         if (layout == MESO_LAYOUT_CELL_MAJOR) {
@@ -1489,6 +1630,7 @@ int meso_hip_set_state(meso_hip_ctx *c, const double *f, int32_t layout, const d
     if (layout != MESO_LAYOUT_VELOCITY_MAJOR && layout != MESO_LAYOUT_CELL_MAJOR)
         return set_err(c, MESO_E_INVALID, "bad layout");
     int rc;
+    if ((rc = set_reference_from(c, rho, c->s.model == MESO_MODEL_CDUGKS_SHAKHOV ? T : nullptr))) return rc;
     for (int k = 0; k < c->nd; ++k)
         if ((rc = transpose(c, true, const_cast<double *>(f), k, layout))) return rc;
     const size_t nc = c->s.nc;

@@ -40,7 +40,10 @@ struct EqParams {
     double q[3];
     double A, B, iRT, P1, P2, hk;
     double mix;  // weight of the equilibrium in the face value (C_s or c_eq)
-    double pad;
+    // Deviation from the reference equilibrium (FP32 storage only):
+    // Shakhov: g_S = g_ref exp(lnAr + dB|xi|^2 + 2B xi.u - B|u|^2) [...],
+    //          dhk = hk - hk_ref;  BGK: lnAr = rho - rho_ref.
+    double lnAr, dB, dhk;
 };
 
 // Parameters of one boundary face after the wall / inlet / outlet stage: the
@@ -116,7 +119,14 @@ __host__ __device__ inline void make_shakhov(EqParams &e, double rho, const doub
     e.P2 = (1.0 - Pr) / (5.0 * p);
     e.hk = (K + 3.0 - D) * RT;
     e.mix = 0.0;
-    e.pad = 0.0;
+    e.lnAr = e.dB = e.dhk = 0.0;
+}
+
+// Fill the deviation terms of e against the reference equilibrium r.
+__host__ __device__ inline void relate_shakhov(EqParams &e, const EqParams &r) {
+    e.lnAr = log(e.A / r.A);
+    e.dB = r.B - e.B;
+    e.dhk = e.hk - r.hk;
 }
 
 __host__ __device__ inline void make_bgk(EqParams &e, double rho, const double u[3], double RT) {
@@ -129,7 +139,39 @@ __host__ __device__ inline void make_bgk(EqParams &e, double rho, const double u
     e.iRT = 1.0 / RT;
     e.P1 = e.P2 = e.hk = 0.0;
     e.mix = 0.0;
-    e.pad = 0.0;
+    e.lnAr = e.dB = e.dhk = 0.0;
+}
+
+__host__ __device__ inline void relate_bgk(EqParams &e, const EqParams &r) { e.lnAr = e.A - r.A; }
+
+// Shakhov pair as deviations from the reference pair (gref, href = hk_ref gref):
+//   g_S - gref = gref [expm1(Dl)(1 + S) + S]
+//   h_S - href = gref [expm1(Dl)(hk + H) + dhk + H]
+// with Dl = lnAr + dB|xi|^2 + 2B xi.u - B|u|^2, so the rounding error scales
+// with the deviation, not with the distribution.
+__device__ __forceinline__ void shakhov_dev(const EqParams &e, const float xi[3], int D, int K, float gref,
+                                            float &gs, float &hs) {
+    const float u0 = float(e.u[0]), u1 = float(e.u[1]), u2 = float(e.u[2]);
+    const float c0 = xi[0] - u0, c1 = xi[1] - u1, c2 = xi[2] - u2;
+    const float cc = c0 * c0 + c1 * c1 + c2 * c2;
+    const float cq = c0 * float(e.q[0]) + c1 * float(e.q[1]) + c2 * float(e.q[2]);
+    const float xx = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+    const float xu = xi[0] * u0 + xi[1] * u1 + xi[2] * u2;
+    const float dl = float(e.lnAr) + float(e.dB) * xx + float(2.0 * e.B) * xu - float(e.B * (e.u[0] * e.u[0] + e.u[1] * e.u[1] + e.u[2] * e.u[2]));
+    const float em1 = expm1f(dl);
+    const float ccr = cc * float(e.iRT);
+    const float S = float(e.P1) * cq * (ccr - float(D) - 2.0f);
+    const float H = float(e.P2) * cq * ((ccr - float(D)) * float(K + 3 - D) - float(2 * K));
+    gs = gref * (em1 * (1.0f + S) + S);
+    hs = gref * (em1 * (float(e.hk) + H) + float(e.dhk) + H);
+}
+
+// BGK polynomial equilibrium minus rho_ref.
+__device__ __forceinline__ float bgk_dev(const EqParams &e, const float xi[3]) {
+    const float u0 = float(e.u[0]), u1 = float(e.u[1]), u2 = float(e.u[2]);
+    const float ku = (xi[0] * u0 + xi[1] * u1 + xi[2] * u2) * float(e.iRT);
+    const float uu = (u0 * u0 + u1 * u1 + u2 * u2) * float(e.iRT);
+    return float(e.lnAr) + float(e.A) * (ku + (ku * ku - uu) * 0.5f);
 }
 
 // tau(rho, T) = mu0 (T/T0)^omega / (rho R T)   (cdugks_shakhov.cpp:181-183)

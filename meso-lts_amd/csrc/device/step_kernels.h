@@ -43,6 +43,7 @@ struct KArgs {
     const double *nb_a;         // [nnz*3] w_j (Cx.dr_j, Cy.dr_j, Cz.dr_j)
     const float *nb_af;
     const double *cell_vol;
+    const float *cell_S;        // [nc*3] sum over the cell's faces of +-A n (0 for a closed, planar-faced cell)
     const int *face_bnd;        // [nf] boundary index or -1
     const int *bnd_face;        // [nb]
     const int *bnd_mark;        // [nb] index into marks
@@ -70,7 +71,47 @@ struct KArgs {
     int *bad_step;
     int *halt;
     int step_index;
+    // FP32 storage keeps f - f_ref; f_ref is the equilibrium `ref` at rest.
+    EqParams ref;
+    const double *fref_d;  // [2][nv] g_ref then h_ref (FP64)
+    const float *fref_f;   // same, FP32
+    double mref[17];       // raw face moments of f_ref (face-moment column layout)
+    double rflux[5][3];    // per flux column k: sum_v w xi psi_k f_ref
 };
+
+// Equilibrium in the stored form: the full value for FP64 storage, the
+// deviation from f_ref for FP32 storage.
+template <int MODEL, int D, class T>
+__device__ __forceinline__ void eq_store(const KArgs &a, const EqParams &e, const T x[3], int v, T &g, T &h) {
+    if constexpr (sizeof(T) == 8) {
+        if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<T>(e, x, D, a.s.K, g, h);
+        else {
+            g = bgk_eq<T>(e, x);
+            h = T(0);
+        }
+    } else {
+        if (MODEL == CDUGKS_SHAKHOV) shakhov_dev(e, x, D, a.s.K, a.fref_f[v], g, h);
+        else {
+            g = bgk_dev(e, x);
+            h = 0.0f;
+        }
+    }
+}
+
+// Full FP64 value of a stored value of distribution k (0: g, 1: h) at velocity v.
+template <class T>
+__device__ __forceinline__ double full_value(const KArgs &a, T stored, int k, int v) {
+    if constexpr (sizeof(T) == 8) return stored;
+    else return double(stored) + a.fref_d[k * a.s.nv + v];
+}
+
+// (1 - m) x + m y, evaluated as x + m (y - x) for FP32 storage so that a
+// weight near 0 or 1 does not bias the result by its own rounding.
+template <class T>
+__device__ __forceinline__ T lerp_store(T x, T y, T m) {
+    if constexpr (sizeof(T) == 8) return (T(1) - m) * x + m * y;
+    else return x + m * (y - x);
+}
 
 template <int MODEL, int D>
 __host__ __device__ constexpr int ncol_face() {

@@ -614,6 +614,7 @@ struct oracle_ctx {
 
     // ---------------- initial() (cdugks_shakhov.cpp:102-168, cdugks.cpp:66-104, dugks.cpp:67-105)
     void init_equilibrium(const double *r0, const double *u0, const double *T0p) {
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < NC; ++c) {
             V3 up(u0 + 3 * c);
             double m0 = 0, m2 = 0;
@@ -650,6 +651,7 @@ struct oracle_ctx {
     }
 
     void init_from_macros(const double *r0, const double *u0, const double *T0p, const double *q0) {
+#pragma omp parallel for schedule(static)
         for (int c = 0; c < NC; ++c) {
             rho[c] = r0[c];
             u[c] = V3(u0 + 3 * c);

@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures under tests/golden/<case>/ from meso itself.
+
+Each fixture directory holds the inputs (case.txt, mesh.neu, dvs.dvs) and
+golden.npz: geometry as meso builds it (cell centroids/volumes, face
+cells/centres/areas/normals, least-squares rows, DVS abscissae/weights, dt)
+and the macroscopic fields after the listed steps (rho, u, and for Shakhov T,
+q and the cell flux moments fluxM0..2), plus the full distribution state after
+`state_step` for the small cases.
+
+The outputs come from oracle/_ref/meso_ref, i.e. meso compiled from its own
+sources by oracle/Makefile.ref (build it first: make -f oracle/Makefile.ref),
+run with one MPI rank. Inputs are generated by meso_hip.meshgen with fixed
+seeds, except re400_cavity and kn0075_cavity, which use meso's shipped
+files/cavity.neu (1928 mixed quad/triangle cells), files/gh-3.dvs,
+files/nc.dvs and the settings of files/case-re400.txt / files/case-kn0.075.txt.
+
+usage: python tests/golden/make_golden.py [--ref-root /root/reference]
+"""
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, os.path.join(ROOT, "meso-lts_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from meso_hip import meshgen as mg  # noqa: E402
+
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "meso_ref")
+
+
+def gen_inputs(name, d, ref_root):
+    mu075 = mg.viscosity_from_kn(0.075)
+    if name == "bgk_quad8":
+        return mg.bgk_cavity_case(d, mg.quad_cavity(8))
+    if name == "dugks_quad8":
+        return mg.bgk_cavity_case(d, mg.quad_cavity(8), solver="dugks@incompressible")
+    if name == "bgk_tri6":
+        return mg.bgk_cavity_case(d, mg.tri_cavity(6, seed=4))
+    if name == "shak_quad6":
+        return mg.shakhov_cavity_case(d, mg.quad_cavity(6, jitter=0.1, seed=3), ("nc", 1, 7, 4.5), mu075, 0.2)
+    if name == "shak_tri6":
+        return mg.shakhov_cavity_case(d, mg.tri_cavity(6, jitter=0.2, seed=1), ("nc", 2, 4, 4.5), mu075, 0.2)
+    if name == "shak_hex3":
+        return mg.shakhov_cavity_case(d, mg.hex_cavity(3, jitter=0.05, seed=2), ("sparse-gh", 4, 0.5),
+                                      mg.viscosity_from_kn(0.1), 0.1)
+    if name in ("shak_channel_pressure", "shak_channel_farfield", "bgk_channel_farfield"):
+        import conftest
+        return conftest.channel_case(d, name)
+    if name in ("re400_cavity", "kn0075_cavity"):
+        os.makedirs(d, exist_ok=True)
+        files = os.path.join(ref_root, "files")
+        shared = os.path.join(HERE, "cavity_mesh")
+        os.makedirs(shared, exist_ok=True)
+        shutil.copy(os.path.join(files, "cavity.neu"), os.path.join(shared, "cavity.neu"))
+        if name == "re400_cavity":
+            shutil.copy(os.path.join(files, "gh-3.dvs"), os.path.join(d, "dvs.dvs"))
+            src = os.path.join(files, "case-re400.txt")
+        else:
+            shutil.copy(os.path.join(files, "nc.dvs"), os.path.join(d, "dvs.dvs"))
+            src = os.path.join(files, "case-kn0.075.txt")
+        lines = []
+        for line in open(src):
+            tok = line.split()
+            if tok and tok[0] == "mesh-file":
+                line = "mesh-file       ../cavity_mesh/cavity.neu\n"
+            elif tok and tok[0] == "dvs-file":
+                line = "dvs-file        dvs.dvs\n"
+            elif tok and tok[0] == "case-name":
+                line = f"case-name       {name}\n"
+            lines.append(line)
+        with open(os.path.join(d, "case.txt"), "w") as fp:
+            fp.writelines(lines)
+        return os.path.join(d, "case.txt")
+    raise KeyError(name)
+
+
+CASES = {
+    # name: (steps, state_step or None)
+    "bgk_quad8": ([1, 2, 10, 100], 2),
+    "dugks_quad8": ([1, 10, 100], None),
+    "bgk_tri6": ([1, 10], None),
+    "shak_quad6": ([1, 2, 10, 50], 2),
+    "shak_tri6": ([1, 10], None),
+    "shak_hex3": ([1, 5], None),
+    "shak_channel_pressure": ([1, 10], None),
+    "shak_channel_farfield": ([1, 10], None),
+    "bgk_channel_farfield": ([1, 10], None),
+    "re400_cavity": ([1, 10, 100], None),
+    "kn0075_cavity": ([1, 2], None),
+}
+
+
+def run_ref(case, steps, state_step):
+    work = tempfile.mkdtemp()
+    try:
+        for f in os.listdir(os.path.dirname(case)):
+            if f in ("case.txt", "mesh.neu", "dvs.dvs"):
+                shutil.copy(os.path.join(os.path.dirname(case), f), work)
+        shutil.copytree(os.path.join(HERE, "cavity_mesh"), os.path.join(os.path.dirname(work), "cavity_mesh"),
+                        dirs_exist_ok=True)
+        out = os.path.join(work, "out")
+        dump = sorted(set([0] + steps + ([state_step] if state_step else [])))
+        cmd = [REF_BIN, "--case", "case.txt", "--out", out, "--dump", ",".join(map(str, dump))]
+        if state_step:
+            cmd.append("-state")
+        r = subprocess.run(cmd, cwd=work, capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        rd = lambda n, dt=np.float64: np.fromfile(os.path.join(out, n), dtype=dt)
+        nc, nf, nv, dim = map(int, open(os.path.join(out, "meta.txt")).read().split())
+        g = dict(nc=nc, nf=nf, nv=nv, dim=dim, steps=np.array(steps),
+                 cell_pos=rd("cell_pos.f64").reshape(nc, 3), cell_vol=rd("cell_vol.f64"),
+                 lsq_C=rd("lsq_C.f64").reshape(nc, 9), nbr_count=rd("nbr_count.i32", np.int32),
+                 face_pos=rd("face_pos.f64").reshape(nf, 3), face_area=rd("face_area.f64"),
+                 face_normal=rd("face_normal.f64").reshape(nf, 3),
+                 face_cells=rd("face_cells.i32", np.int32).reshape(nf, 2),
+                 dvs_xi=rd("dvs_xi.f64").reshape(nv, 3), dvs_w=rd("dvs_w.f64"), dt=rd("dt.f64")[0],
+                 mesh_scalars=rd("mesh_scalars.f64"))
+        for s in [0] + steps:
+            for k, shape in (("rho", (nc,)), ("u", (nc, 3)), ("T", (nc,)), ("q", (nc, 3)), ("fluxM0", (nc,)),
+                             ("fluxM1", (nc, 3)), ("fluxM2", (nc,))):
+                p = os.path.join(out, f"step{s}_{k}.f64")
+                if os.path.exists(p):
+                    g[f"step{s}_{k}"] = np.fromfile(p, dtype=np.float64).reshape(shape)
+        if state_step:
+            for k in ("f", "g", "h"):
+                p = os.path.join(out, f"step{state_step}_{k}.f64")
+                if os.path.exists(p):
+                    g[f"state{state_step}_{k}"] = np.fromfile(p, dtype=np.float64).reshape(nv, nc)
+            g["state_step"] = state_step
+        return g
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref-root", default="/root/reference")
+    ap.add_argument("cases", nargs="*")
+    args = ap.parse_args()
+    if not os.path.exists(REF_BIN):
+        sys.exit(f"{REF_BIN} missing: run make -f oracle/Makefile.ref first")
+    for name in args.cases or CASES:
+        steps, state_step = CASES[name]
+        d = os.path.join(HERE, name)
+        case = gen_inputs(name, d, args.ref_root)
+        g = run_ref(case, steps, state_step)
+        np.savez_compressed(os.path.join(d, "golden.npz"), **g)
+        print(f"{name}: NC={g['nc']} NF={g['nf']} NV={g['nv']} steps={steps}", flush=True)
+
+
+if __name__ == "__main__":
+    main()
