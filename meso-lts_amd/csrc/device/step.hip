@@ -148,94 +148,226 @@ __device__ __forceinline__ void bnd_value(const KArgs &a, const BndParams &P, in
 }
 
 // ================================================================ kernels
-// (1) raw moments of f-bar+ per (face, velocity tile)
-template <int MODEL, int D, class T, int LP>
+// Sign of n0 . xi against the +-VSMALL dead-band (meso's upwind test,
+// cdugks_shakhov.cpp:227-254). FP64 storage evaluates it in FP64 exactly as
+// meso does; FP32 storage uses the FP32 product when it is clearly away from
+// zero (its rounding error is below 1e-5 for |xi| < 50) and falls back to the
+// FP64 product otherwise, so both paths take the same branch as meso.
+// Returns +1 (owner upwind), -1 (neighbour upwind) or 0 (dead-band), and the
+// product in sn.
+template <class T>
+__device__ __forceinline__ int upwind(const KArgs &a, int f, int v, const T x[3], T &sn) {
+    if constexpr (sizeof(T) == 4) {
+        const float s = a.face_nf[3 * f] * x[0] + a.face_nf[3 * f + 1] * x[1] + a.face_nf[3 * f + 2] * x[2];
+        sn = s;
+        if (fabsf(s) > 1e-4f) return s > 0.0f ? 1 : -1;
+    }
+    const double d = ndot(a, f, v);
+    sn = T(d);
+    return d >= VSMALL ? 1 : (d < -VSMALL ? -1 : 0);
+}
+
+// LSQ gradient of cell c at velocity v from the padded per-cell tables.
+template <int D, class T>
+__device__ __forceinline__ void grad_tab(const KArgs &a, const T *__restrict__ fd, int c, int v, T fc, T gr[3]) {
+    constexpr int MAXN = D == 2 ? 4 : 6;
+    gr[0] = gr[1] = gr[2] = T(0);
+    const int n = a.lsq_cnt[c];
+    const int nvp = a.s.nvp;
+#pragma unroll
+    for (int j = 0; j < MAXN; ++j) {
+        if (j < n) {
+            const int nb = a.lsq_nb[c * MAXN + j];
+            const T df = fd[size_t(nb) * nvp + v] - fc;
+            if constexpr (sizeof(T) == 8) {
+                const double *aj = a.lsq_a + 3 * (c * MAXN + j);
+                gr[0] += aj[0] * df;
+                gr[1] += aj[1] * df;
+                if (D == 3) gr[2] += aj[2] * df;
+            } else {
+                const float *aj = a.lsq_af + 3 * (c * MAXN + j);
+                gr[0] += aj[0] * df;
+                gr[1] += aj[1] * df;
+                if (D == 3) gr[2] += aj[2] * df;
+            }
+        }
+    }
+}
+
+template <int D, class T>
+__device__ __forceinline__ T dot_ext(const T d0, const T d1, const T d2, const T xh[3], const T gr[3]) {
+    T r = (d0 - xh[0]) * gr[0] + (d1 - xh[1]) * gr[1];
+    if (D == 3) r += (d2 - xh[2]) * gr[2];
+    return r;
+}
+
+// Gradient of one stencil group from its loaded values: sum_j a_j (f_j - f_0).
+template <int D, int MAXN, class T>
+__device__ __forceinline__ void group_grad(const T *vals, const T *__restrict__ av, T gr[3]) {
+    gr[0] = gr[1] = gr[2] = T(0);
+#pragma unroll
+    for (int j = 0; j < MAXN; ++j) {
+        const T df = vals[1 + j] - vals[0];
+        gr[0] += av[3 * j] * df;
+        gr[1] += av[3 * j + 1] * df;
+        if (D == 3) gr[2] += av[3 * j + 2] * df;
+    }
+}
+
+template <class T>
+__device__ __forceinline__ const T *coef_ptr(const double *d, const float *f, size_t off) {
+    if constexpr (sizeof(T) == 8) return reinterpret_cast<const T *>(d + off);
+    else return reinterpret_cast<const T *>(f + off);
+}
+
+// (1) raw moments of f-bar+ per (face, velocity tile); one wave per pair.
+// All stencil values of an iteration are loaded before any is used.
+template <int MODEL, int D, class T>
 __global__ __launch_bounds__(BLOCK) void k_face_moments(KArgs a) {
     if (*a.bad_count > 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) *a.halt = 1;
         return;
     }
-    constexpr int IPB = BLOCK / LP;
+    constexpr int WPB = BLOCK / 64;
+    constexpr int MAXN = D == 2 ? 4 : 6;
+    constexpr int GS = 1 + MAXN;
     constexpr int NCOL = ncol_face<MODEL, D>();
+    constexpr bool SH = MODEL == CDUGKS_SHAKHOV;
     const int nf = a.s.nf;
-    const int nblk = (nf + IPB - 1) / IPB;
+    const int nblk = (nf + WPB - 1) / WPB;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int tile = b / nblk, fb = b - tile * nblk;
-    const int slot = threadIdx.x / LP, li = threadIdx.x % LP;
-    const int f = fb * IPB + slot;
+    const int lane = threadIdx.x % 64;
+    const int f = __builtin_amdgcn_readfirstlane(fb * WPB + int(threadIdx.x / 64));
+    if (f >= nf) return;
+    const int nvp = a.s.nvp;
+    const T hd = T(a.s.half_dt);
+    const T *fg = (const T *)a.fin_g;
+    const T *fh = (const T *)a.fin_h;
+    const int *ids = a.fs_id + size_t(f) * 2 * GS;
+    const T *av = coef_ptr<T>(a.fs_a, a.fs_af, size_t(f) * 2 * MAXN * 3);
+    T d[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d[k] = face_d<T>(a, f, k / 3, k % 3);
     double acc[NCOL];
 #pragma unroll
     for (int k = 0; k < NCOL; ++k) acc[k] = 0.0;
-    if (f < nf) {
-        const int2 oc = a.face_cells[f];
-        const T hd = T(a.s.half_dt);
-        const T *fg = (const T *)a.fin_g;
-        const T *fh = (const T *)a.fin_h;
-        const int v0 = tile * a.s.vt_f, v1 = min(v0 + a.s.vt_f, a.s.nv);
-        for (int v = v0 + li; v < v1; v += LP) {
-            T x[3];
-            load_xi<T>(a, v, x);
-            T xh[3] = {x[0] * hd, x[1] * hd, x[2] * hd};
-            const double sn = ndot(a, f, v);
-            const double X0 = a.xi[3 * v], X1 = a.xi[3 * v + 1], X2 = a.xi[3 * v + 2];
-            const double wv = a.w[v];
-            const double wg = wv * double(fbar_at<D, T>(a, fg, f, oc.x, oc.y, sn, v, xh));
-            const double wgx = wg * X0, wgy = wg * X1, wgz = wg * X2;
-            acc[Col<D>::M0] += wg;
-            acc[Col<D>::M1] += wgx;
-            acc[Col<D>::M1 + 1] += wgy;
-            if (D == 3) acc[Col<D>::M1 + 2] += wgz;
-            if (D == 2) {
-                acc[Col<D>::M2] += wgx * X0;
-                acc[Col<D>::M2 + 1] += wgx * X1;
-                acc[Col<D>::M2 + 2] += wgy * X1;
-            } else {
-                acc[Col<D>::M2] += wgx * X0;
-                acc[Col<D>::M2 + 1] += wgx * X1;
-                acc[Col<D>::M2 + 2] += wgx * X2;
-                acc[Col<D>::M2 + 3] += wgy * X1;
-                acc[Col<D>::M2 + 4] += wgy * X2;
-                acc[Col<D>::M2 + 5] += wgz * X2;
+    const int v0 = tile * a.s.vt_f, v1 = min(v0 + a.s.vt_f, a.s.nv);
+    for (int v = v0 + lane; v < v1; v += 64) {
+        T vg[2 * GS], vh[2 * GS];
+#pragma unroll
+        for (int k = 0; k < 2 * GS; ++k) {
+            const size_t r = size_t(ids[k]) * nvp + v;
+            vg[k] = fg[r];
+            if (SH) vh[k] = fh[r];
+        }
+        const double4 X = a.vel_d[v];
+        T x[3];
+        if constexpr (sizeof(T) == 8) {
+            x[0] = X.x;
+            x[1] = X.y;
+            x[2] = X.z;
+        } else {
+            const float4 xf = a.vel_f[v];
+            x[0] = xf.x;
+            x[1] = xf.y;
+            x[2] = xf.z;
+        }
+        const T xh[3] = {x[0] * hd, x[1] * hd, x[2] * hd};
+        T sn;
+        const int up = upwind<T>(a, f, v, x, sn);
+        T go_g[3] = {T(0), T(0), T(0)}, go_h[3] = {T(0), T(0), T(0)};
+        T gn_g[3] = {T(0), T(0), T(0)}, gn_h[3] = {T(0), T(0), T(0)};
+        if (a.s.grad) {
+            if (__any(up >= 0)) {
+                group_grad<D, MAXN, T>(vg, av, go_g);
+                if (SH) group_grad<D, MAXN, T>(vh, av, go_h);
             }
-            if (MODEL == CDUGKS_SHAKHOV) {
-                const double wh = wv * double(fbar_at<D, T>(a, fh, f, oc.x, oc.y, sn, v, xh));
-                const double wgk = wg * (X0 * X0 + X1 * X1 + X2 * X2);
-                acc[Col<D>::M3] += wgk * X0;
-                acc[Col<D>::M3 + 1] += wgk * X1;
-                if (D == 3) acc[Col<D>::M3 + 2] += wgk * X2;
-                acc[Col<D>::H0] += wh;
-                acc[Col<D>::H1] += wh * X0;
-                acc[Col<D>::H1 + 1] += wh * X1;
-                if (D == 3) acc[Col<D>::H1 + 2] += wh * X2;
+            if (__any(up <= 0)) {
+                group_grad<D, MAXN, T>(vg + GS, av + 3 * MAXN, gn_g);
+                if (SH) group_grad<D, MAXN, T>(vh + GS, av + 3 * MAXN, gn_h);
             }
+        }
+        const T Xog = dot_ext<D, T>(d[0], d[1], d[2], xh, go_g), Xng = dot_ext<D, T>(d[3], d[4], d[5], xh, gn_g);
+        const T gb = up > 0 ? vg[0] + Xog : (up < 0 ? vg[GS] + Xng : T(0.5) * (vg[0] + vg[GS] + Xog + Xng));
+        T hb = T(0);
+        if (SH) {
+            const T Xoh = dot_ext<D, T>(d[0], d[1], d[2], xh, go_h), Xnh = dot_ext<D, T>(d[3], d[4], d[5], xh, gn_h);
+            hb = up > 0 ? vh[0] + Xoh : (up < 0 ? vh[GS] + Xnh : T(0.5) * (vh[0] + vh[GS] + Xoh + Xnh));
+        }
+        const double X0 = X.x, X1 = X.y, X2 = X.z;
+        const double wv = X.w;
+        const double wg = wv * double(gb);
+        const double wgx = wg * X0, wgy = wg * X1, wgz = wg * X2;
+        acc[Col<D>::M0] += wg;
+        acc[Col<D>::M1] += wgx;
+        acc[Col<D>::M1 + 1] += wgy;
+        if (D == 3) acc[Col<D>::M1 + 2] += wgz;
+        if (D == 2) {
+            acc[Col<D>::M2] += wgx * X0;
+            acc[Col<D>::M2 + 1] += wgx * X1;
+            acc[Col<D>::M2 + 2] += wgy * X1;
+        } else {
+            acc[Col<D>::M2] += wgx * X0;
+            acc[Col<D>::M2 + 1] += wgx * X1;
+            acc[Col<D>::M2 + 2] += wgx * X2;
+            acc[Col<D>::M2 + 3] += wgy * X1;
+            acc[Col<D>::M2 + 4] += wgy * X2;
+            acc[Col<D>::M2 + 5] += wgz * X2;
+        }
+        if (SH) {
+            const double wh = wv * double(hb);
+            const double wgk = wg * (X0 * X0 + X1 * X1 + X2 * X2);
+            acc[Col<D>::M3] += wgk * X0;
+            acc[Col<D>::M3 + 1] += wgk * X1;
+            if (D == 3) acc[Col<D>::M3 + 2] += wgk * X2;
+            acc[Col<D>::H0] += wh;
+            acc[Col<D>::H1] += wh * X0;
+            acc[Col<D>::H1 + 1] += wh * X1;
+            if (D == 3) acc[Col<D>::H1 + 2] += wh * X2;
         }
     }
 #pragma unroll
-    for (int k = 0; k < NCOL; ++k) acc[k] = seg_sum<LP>(acc[k]);
-    if (f < nf && li == 0) {
+    for (int k = 0; k < NCOL; ++k) acc[k] = seg_sum<64>(acc[k]);
+    if (lane == 0) {
         double *dst = a.part_f + (size_t(f) * a.s.ntile_f + tile) * NCOL;
 #pragma unroll
         for (int k = 0; k < NCOL; ++k) dst[k] = acc[k];
     }
 }
 
-// (2) face macros, equilibrium constants and flux moments of interior faces
-template <int MODEL, int D, class T, int LP>
+// Axis-table entry t of equilibrium e in the storage type's form.
+template <class T>
+__device__ __forceinline__ T tab_entry(const KArgs &a, const EqParams &e, int t) {
+    const double X = a.tab_x[t];
+    const int ax = a.tab_ax[t];
+    const double c = X - (ax == 0 ? e.u[0] : (ax == 1 ? e.u[1] : e.u[2]));
+    if constexpr (sizeof(T) == 8) return exp(-e.B * c * c);
+    else return T(expm1(a.ref.B * X * X - e.B * c * c));
+}
+
+// (2) face macros, equilibrium constants, axis tables and flux moments of the
+// mixed value of interior faces; one wave per face
+template <int MODEL, int D, class T, bool TAB>
 __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
+    extern __shared__ unsigned char lds_raw[];
     if (*a.halt) return;
-    constexpr int IPB = BLOCK / LP;
+    constexpr int WPB = BLOCK / 64;
     constexpr int NCOL = ncol_face<MODEL, D>();
     constexpr int NFL = D + 2;
     const int nf = a.s.nf;
-    const int slot = threadIdx.x / LP, li = threadIdx.x % LP;
-    const int f = xcd_block(blockIdx.x, gridDim.x) * IPB + slot;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int f = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * WPB + wave);
     const bool act = f < nf;
+    T *tb = reinterpret_cast<T *>(lds_raw) + size_t(wave) * a.nt;
     EqParams e;
+    EqF ef;
     double Fb[NFL];
     double acc[NFL];
 #pragma unroll
     for (int k = 0; k < NFL; ++k) acc[k] = Fb[k] = 0.0;
     bool interior = false;
+    double n[3] = {0.0, 0.0, 0.0};
     if (act) {
         double M[NCOL];
 #pragma unroll
@@ -248,7 +380,7 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
 #pragma unroll
             for (int k = 0; k < NCOL; ++k) M[k] += a.mref[k];
         }
-        const double n[3] = {a.face_n[3 * f], a.face_n[3 * f + 1], a.face_n[3 * f + 2]};
+        for (int i = 0; i < 3; ++i) n[i] = a.face_n[3 * f + i];
         const double rho = M[Col<D>::M0];
         const double inv = 1.0 / rho;
         double u[3] = {M[Col<D>::M1] * inv, M[Col<D>::M1 + 1] * inv, D == 3 ? M[Col<D>::M1 + 2] * inv : 0.0};
@@ -281,10 +413,10 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
             double q[3];
             const double cq = tau / (2.0 * tau + hdt * a.s.Pr);
             for (int i = 0; i < 3; ++i) {
-                double M3i = i < D ? M[Col<D>::M3 + i] : 0.0;
-                double H1i = i < D ? M[Col<D>::H1 + i] : 0.0;
-                double m3 = M3i - 2.0 * Mu[i] + uu * M1[i] - u[i] * trM2 + 2.0 * u[i] * M1u -
-                            u[i] * uu * rho + H1i - u[i] * H0;
+                const double M3i = i < D ? M[Col<D>::M3 + i] : 0.0;
+                const double H1i = i < D ? M[Col<D>::H1 + i] : 0.0;
+                const double m3 = M3i - 2.0 * Mu[i] + uu * M1[i] - u[i] * trM2 + 2.0 * u[i] * M1u -
+                                  u[i] * uu * rho + H1i - u[i] * H0;
                 q[i] = cq * m3;
             }
             make_shakhov(e, rho, u, T_, q, a.s.R, a.s.Pr, D, a.s.K);
@@ -301,39 +433,65 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
             if constexpr (sizeof(T) == 4) relate_bgk(e, a.ref);
             e.mix = hdt / (2.0 * a.s.tau + hdt);
         }
-        if (li == 0) a.face_eq[f] = e;
-        interior = a.face_bnd[f] < 0;
-        if (interior) {
-            const int nv = a.s.nv;
-            for (int v = li; v < nv; v += LP) {
-                T x[3];
-                load_xi<T>(a, v, x);
-                const double X0 = a.xi[3 * v], X1 = a.xi[3 * v + 1], X2 = a.xi[3 * v + 2];
-                const double wsn = a.w[v] * (n[0] * X0 + n[1] * X1 + n[2] * X2);
-                T gs, hs;
-                eq_store<MODEL, D, T>(a, e, x, v, gs, hs);
-                const double wg = wsn * double(gs);
-                acc[0] += wg;
-                acc[1] += wg * X0;
-                acc[2] += wg * X1;
-                if (D == 3) acc[3] += wg * X2;
-                if (MODEL == CDUGKS_SHAKHOV) acc[D + 1] += wg * (X0 * X0 + X1 * X1 + X2 * X2) + wsn * double(hs);
+        if constexpr (sizeof(T) == 4) {
+            ef = to_eqf(e);
+            if (MODEL != CDUGKS_SHAKHOV) ef.hk = float(e.A);
+        }
+        if (lane == 0) {
+            a.face_eq[f] = e;
+            if constexpr (sizeof(T) == 4) a.face_eqf[f] = ef;
+        }
+        if (TAB) {
+            T *gt = reinterpret_cast<T *>(a.face_tab) + size_t(f) * a.nt;
+            for (int t = lane; t < a.nt; t += 64) {
+                const T val = tab_entry<T>(a, e, t);
+                tb[t] = val;
+                gt[t] = val;
             }
         }
+        interior = a.face_bnd[f] < 0;
     }
-    // the whole LP segment shares `act` and `interior`
-#pragma unroll
-    for (int k = 0; k < NFL; ++k) acc[k] = seg_sum<LP>(acc[k]);
-    if (act && interior && li == 0) {
-        if constexpr (sizeof(T) == 4) {  // flux moments of f_ref along n
-            const double n[3] = {a.face_n[3 * f], a.face_n[3 * f + 1], a.face_n[3 * f + 2]};
-            for (int k = 0; k < NFL; ++k)
-                acc[k] += n[0] * a.rflux[k][0] + n[1] * a.rflux[k][1] + n[2] * a.rflux[k][2];
+    if (TAB) __syncthreads();
+    if (act && interior) {
+        const int nv = a.s.nv;
+        for (int v = lane; v < nv; v += 64) {
+            const double4 X = a.vel_d[v];
+            const double wsn = X.w * (n[0] * X.x + n[1] * X.y + n[2] * X.z);
+            T gs, hs = T(0);
+            if (MODEL == CDUGKS_SHAKHOV && TAB) {
+                if constexpr (sizeof(T) == 4) {
+                    const float4 xf = a.vel_f[v];
+                    const float x[3] = {xf.x, xf.y, xf.z};
+                    shakhov_tab_f(ef, reinterpret_cast<const float *>(tb), a.vtab[v], x, D, a.s.K, a.fref_f2[v].x, gs,
+                                  hs);
+                } else {
+                    const double x[3] = {X.x, X.y, X.z};
+                    shakhov_tab_d(e, reinterpret_cast<const double *>(tb), a.vtab[v], x, D, a.s.K, gs, hs);
+                }
+            } else {
+                T x[3];
+                load_xi<T>(a, v, x);
+                eq_store<MODEL, D, T>(a, e, x, v, gs, hs);
+            }
+            const double wg = wsn * double(gs);
+            acc[0] += wg;
+            acc[1] += wg * X.x;
+            acc[2] += wg * X.y;
+            if (D == 3) acc[3] += wg * X.z;
+            if (MODEL == CDUGKS_SHAKHOV) acc[D + 1] += wg * (X.x * X.x + X.y * X.y + X.z * X.z) + wsn * double(hs);
         }
-        const double mix = e.mix;
-        const int nfl = MODEL == CDUGKS_SHAKHOV ? D + 2 : D + 1;
-        for (int k = 0; k < nfl; ++k) a.face_flux[size_t(f) * NFL + k] = (1.0 - mix) * Fb[k] + mix * acc[k];
-        if (MODEL != CDUGKS_SHAKHOV) a.face_flux[size_t(f) * NFL + D + 1] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NFL; ++k) acc[k] = seg_sum<64>(acc[k]);
+        if (lane == 0) {
+            if constexpr (sizeof(T) == 4) {  // flux moments of f_ref along n
+                for (int k = 0; k < NFL; ++k)
+                    acc[k] += n[0] * a.rflux[k][0] + n[1] * a.rflux[k][1] + n[2] * a.rflux[k][2];
+            }
+            const double mix = e.mix;
+            const int nfl = MODEL == CDUGKS_SHAKHOV ? D + 2 : D + 1;
+            for (int k = 0; k < nfl; ++k) a.face_flux[size_t(f) * NFL + k] = (1.0 - mix) * Fb[k] + mix * acc[k];
+            if (MODEL != CDUGKS_SHAKHOV) a.face_flux[size_t(f) * NFL + D + 1] = 0.0;
+        }
     }
 }
 
@@ -555,58 +713,68 @@ __device__ __forceinline__ bool cell_macros(const KArgs &a, int c, CellStep<MODE
     }
 }
 
-// Mixed value of interior face f at velocity v seen from cell c, whose
-// f-bar+ side is extrapolated with gc (c's own gradient) or the other cell's.
-template <int D, class T>
-__device__ __forceinline__ T interior_fbar(const KArgs &a, const T *__restrict__ fd, int f, int2 oc, int c, double sn,
-                                           int v, const T xh[3], T fc, const T gc[3]) {
-    const int nvp = a.s.nvp;
-    const int side_c = (oc.x == c) ? 0 : 1;
-    const int other = side_c == 0 ? oc.y : oc.x;
-    if (sn >= VSMALL || sn < -VSMALL) {
-        const int up_side = (sn >= VSMALL) ? 0 : 1;
-        if (up_side == side_c) return a.s.grad ? fc + extrap<D, T>(a, f, side_c, xh, gc) : fc;
-        const T fo = fd[size_t(other) * nvp + v];
-        if (!a.s.grad) return fo;
-        T go[3];
-        lsq_grad<D, T>(a, fd, other, v, fo, go);
-        return fo + extrap<D, T>(a, f, up_side, xh, go);
+// Face equilibrium pair in the stored form at velocity x (slot v).
+template <int MODEL, int D, class T>
+__device__ __forceinline__ void face_eq_store(const KArgs &a, int f, const T x[3], int v, T &gs, T &hs) {
+    if constexpr (sizeof(T) == 4) {
+        const EqF &e = a.face_eqf[f];
+        if (MODEL == CDUGKS_SHAKHOV) shakhov_devf(e, x, D, a.s.K, a.fref_f[v], gs, hs);
+        else {
+            gs = bgk_devf(e, x);
+            hs = 0.0f;
+        }
+    } else {
+        eq_store<MODEL, D, T>(a, a.face_eq[f], x, v, gs, hs);
     }
-    const T fo = fd[size_t(other) * nvp + v];
-    const T f_own = side_c == 0 ? fc : fo, f_nei = side_c == 0 ? fo : fc;
-    if (!a.s.grad) return T(0.5) * (f_own + f_nei);
-    T go[3];
-    lsq_grad<D, T>(a, fd, other, v, fo, go);
-    const T X_own = side_c == 0 ? extrap<D, T>(a, f, 0, xh, gc) : extrap<D, T>(a, f, 0, xh, go);
-    const T X_nei = side_c == 0 ? extrap<D, T>(a, f, 1, xh, go) : extrap<D, T>(a, f, 1, xh, gc);
-    return T(0.5) * (f_own + f_nei + X_own + X_nei);
 }
 
-// (4) cell flux + update per (cell, velocity tile)
-template <int MODEL, int D, class T, int LP>
-__global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a) {
+template <int MODEL, int D, class T>
+__device__ __forceinline__ void cell_eq_store(const KArgs &a, const EqParams &e, const EqF &ef, const T x[3], int v,
+                                              T &gs, T &hs) {
+    if constexpr (sizeof(T) == 4) {
+        if (MODEL == CDUGKS_SHAKHOV) shakhov_devf(ef, x, D, a.s.K, a.fref_f[v], gs, hs);
+        else {
+            gs = bgk_devf(ef, x);
+            hs = 0.0f;
+        }
+    } else {
+        eq_store<MODEL, D, T>(a, e, x, v, gs, hs);
+    }
+}
+
+// (4) cell flux + update; one wave per (cell, velocity tile). BND selects the
+// cells that have a boundary face (launched separately, so the interior
+// kernel carries no boundary code). TAB: tensor-product DVS with axis tables.
+template <int MODEL, int D, class T, bool TAB, bool BND>
+__global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, int list_count) {
+    extern __shared__ unsigned char lds_raw[];
     if (*a.halt) return;
-    constexpr int IPB = BLOCK / LP;
-    const int nc = a.s.nc;
-    const int nblk = (nc + IPB - 1) / IPB;
+    constexpr int WPB = BLOCK / 64;
+    constexpr int MAXF = D == 2 ? 4 : 6;
+    constexpr int MAXN = MAXF;
+    constexpr int GS = 1 + MAXN, NG = 1 + MAXF;
+    constexpr bool SH = MODEL == CDUGKS_SHAKHOV;
+    const int nblk = (list_count + WPB - 1) / WPB;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int tile = b / nblk, cb = b - tile * nblk;
-    const int slot = threadIdx.x / LP, li = threadIdx.x % LP;
-    const int c = cb * IPB + slot;
-    const bool act = c < nc;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int li = cb * WPB + wave;
+    const bool act = li < list_count;
+    const int c = __builtin_amdgcn_readfirstlane(act ? a.cell_list[list_begin + li] : 0);
+    T *tabs = reinterpret_cast<T *>(lds_raw) + size_t(wave) * (MAXF + 2) * a.nt;
+    CellStep<MODEL, D> cs;
+    int nfc = 0;
     if (act) {
-        CellStep<MODEL, D> cs;
         const bool ok = cell_macros<MODEL, D, T>(a, c, cs);
-        if (tile == 0 && li == 0) {
+        if (tile == 0 && lane == 0) {
             if (MODEL != DUGKS_BGK) {
                 a.rho_out[c] = cs.rho;
                 for (int i = 0; i < 3; ++i) a.u_out[3 * c + i] = cs.u[i];
-                if (MODEL == CDUGKS_SHAKHOV) a.T_out[c] = cs.T;
+                if (SH) a.T_out[c] = cs.T;
                 for (int k = 0; k < 5; ++k) a.fm[5 * c + k] = 0.0;
                 a.fm[5 * c] = cs.phi[0];
                 for (int i = 0; i < D; ++i) a.fm[5 * c + 1 + i] = cs.phi[1 + i];
-                if (MODEL == CDUGKS_SHAKHOV) a.fm[5 * c + 4] = cs.phi[D + 1];
+                if (SH) a.fm[5 * c + 4] = cs.phi[D + 1];
             }
             if (!ok) {
                 atomicAdd(a.bad_count, 1);
@@ -614,140 +782,270 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a) {
                 atomicMin(a.bad_step, a.step_index);
             }
         }
-        const T *fg = (const T *)a.fin_g;
-        const T *fh = (const T *)a.fin_h;
-        T *og = (T *)a.fout_g;
-        T *oh = (T *)a.fout_h;
-        const T hd = T(a.s.half_dt);
-        const T dtv = T(cs.dt_v);
-        const size_t row = size_t(c) * a.s.nvp;
-        const int j0 = a.cf_ptr[c], j1 = a.cf_ptr[c + 1];
-        const int v0 = tile * a.s.vt_c, v1 = min(v0 + a.s.vt_c, a.s.nv);
-        for (int v = v0 + li; v < v1; v += LP) {
-            T x[3];
-            load_xi<T>(a, v, x);
-            T xh[3] = {x[0] * hd, x[1] * hd, x[2] * hd};
-            const T fcg = fg[row + v];
-            T gcg[3] = {T(0), T(0), T(0)};
-            if (a.s.grad) lsq_grad<D, T>(a, fg, c, v, fcg, gcg);
-            T fch = T(0), gch[3] = {T(0), T(0), T(0)};
-            if (MODEL == CDUGKS_SHAKHOV) {
-                fch = fh[row + v];
-                if (a.s.grad) lsq_grad<D, T>(a, fh, c, v, fch, gch);
+        nfc = a.cs_nf[c];
+        if (TAB) {
+            const int nt = a.nt;
+            for (int k = 0; k < nfc; ++k) {
+                const T *src = reinterpret_cast<const T *>(a.face_tab) + size_t(a.cs_face[c * MAXF + k]) * nt;
+                for (int t = lane; t < nt; t += 64) tabs[k * nt + t] = src[t];
             }
-            T Fg = T(0), Fh = T(0);
-            for (int j = j0; j < j1; ++j) {
-                const int f = a.cf_ids[j];
-                const int2 oc = a.face_cells[f];
-                const double sn = ndot(a, f, v);
-                const EqParams &ef = a.face_eq[f];
-                const T mix = T(ef.mix);
+            for (int t = lane; t < nt; t += 64) {
+                tabs[MAXF * nt + t] = tab_entry<T>(a, cs.en, t);
+                tabs[(MAXF + 1) * nt + t] = tab_entry<T>(a, cs.e, t);
+            }
+        }
+    }
+    if (TAB) __syncthreads();
+    if (!act) return;
+    EqF en_f, e_f;
+    if constexpr (sizeof(T) == 4) {
+        en_f = to_eqf(cs.en);
+        e_f = to_eqf(cs.e);
+        if (!SH) {
+            en_f.hk = float(cs.en.A);
+            e_f.hk = float(cs.e.A);
+        }
+    }
+    const T *fg = (const T *)a.fin_g;
+    const T *fh = (const T *)a.fin_h;
+    T *og = (T *)a.fout_g;
+    T *oh = (T *)a.fout_h;
+    const int nvp = a.s.nvp;
+    const int nt = a.nt;
+    const T hd = T(a.s.half_dt);
+    const T dtv = T(cs.dt_v);
+    const size_t row = size_t(c) * nvp;
+    const int *sid = a.st_id + size_t(c) * NG * GS;
+    const T *sa = coef_ptr<T>(a.st_a, a.st_af, size_t(c) * NG * MAXN * 3);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const int v0 = tile * a.s.vt_c, v1 = min(v0 + a.s.vt_c, a.s.nv);
+    for (int v = v0 + lane; v < v1; v += 64) {
+        // every stencil value this velocity needs, loaded before any use
+        T vg[NG * GS], vh[NG * GS];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            if (g <= nfc) {
+#pragma unroll
+                for (int j = 0; j < GS; ++j) {
+                    const size_t r = size_t(sid[g * GS + j]) * nvp + v;
+                    vg[g * GS + j] = fg[r];
+                    if (SH) vh[g * GS + j] = fh[r];
+                }
+            }
+        }
+        const double4 X = a.vel_d[v];
+        T x[3];
+        if constexpr (sizeof(T) == 8) {
+            x[0] = X.x;
+            x[1] = X.y;
+            x[2] = X.z;
+        } else {
+            const float4 xf = a.vel_f[v];
+            x[0] = xf.x;
+            x[1] = xf.y;
+            x[2] = xf.z;
+        }
+        const int vt = TAB ? a.vtab[v] : 0;
+        T gref = T(0);
+        if constexpr (sizeof(T) == 4) gref = a.fref_f2[v].x;
+        const T xh[3] = {x[0] * hd, x[1] * hd, x[2] * hd};
+        const T fcg = vg[0], fch = SH ? vh[0] : T(0);
+        T gcg[3] = {T(0), T(0), T(0)}, gch[3] = {T(0), T(0), T(0)};
+        if (a.s.grad) {
+            group_grad<D, MAXN, T>(vg, sa, gcg);
+            if (SH) group_grad<D, MAXN, T>(vh, sa, gch);
+        }
+        T Fg = T(0), Fh = T(0);
+#pragma unroll
+        for (int k = 0; k < MAXF; ++k) {
+            if (k < nfc) {
+                const int idx = c * MAXF + k;
+                const int f = a.cs_face[idx];
+                T sn;
+                const int up = upwind<T>(a, f, v, x, sn);
+                T gs, hs = T(0);
+                if (SH && TAB) {
+                    if constexpr (sizeof(T) == 4)
+                        shakhov_tab_f(a.face_eqf[f], reinterpret_cast<const float *>(tabs) + k * nt, vt, x, D, a.s.K,
+                                      gref, gs, hs);
+                    else
+                        shakhov_tab_d(a.face_eq[f], reinterpret_cast<const double *>(tabs) + k * nt, vt, x, D, a.s.K,
+                                      gs, hs);
+                } else {
+                    face_eq_store<MODEL, D, T>(a, f, x, v, gs, hs);
+                }
+                T mix;
+                if constexpr (sizeof(T) == 4) mix = a.face_eqf[f].mix;
+                else mix = a.face_eq[f].mix;
+                T dc[3], dn[3];
+                if constexpr (sizeof(T) == 8) {
+                    for (int i = 0; i < 3; ++i) {
+                        dc[i] = a.cs_d[6 * idx + i];
+                        dn[i] = a.cs_d[6 * idx + 3 + i];
+                    }
+                } else {
+                    for (int i = 0; i < 3; ++i) {
+                        dc[i] = a.cs_df[6 * idx + i];
+                        dn[i] = a.cs_df[6 * idx + 3 + i];
+                    }
+                }
                 T gv, hv = T(0);
-                if (oc.x == oc.y) {
-                    const int bi = a.face_bnd[f];
-                    T gmx, hmx;
-                    bnd_mix_pair<MODEL, D, T>(a, f, c, v, ef, gmx, hmx);
+                const bool bface = BND && a.cs_other[idx] < 0;
+                if (bface) {
+                    // boundary face: owner == neighbour == c
+                    const T Xg = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gcg);
+                    const T bg = up != 0 ? fcg + Xg : T(0.5) * (fcg + fcg + Xg + Xg);
+                    const T gmx = lerp_store<T>(bg, gs, mix);
+                    T hmx = T(0);
+                    if (SH) {
+                        const T Xh = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gch);
+                        const T bh = up != 0 ? fch + Xh : T(0.5) * (fch + fch + Xh + Xh);
+                        hmx = lerp_store<T>(bh, hs, mix);
+                    }
+                    const int bi = a.cs_bnd[idx];
                     if (bi >= 0) {
-                        const BndParams P = a.bnd[bi];
-                        bnd_value<MODEL, D, T>(a, P, f, c, v, sn, ef, gmx, hmx, gv, hv);
+                        bnd_value<MODEL, D, T>(a, a.bnd[bi], f, c, v, double(sn), a.face_eq[f], gmx, hmx, gv, hv);
                     } else {
                         gv = gmx;
                         hv = hmx;
                     }
                 } else {
-                    T gs, hs;
-                    eq_store<MODEL, D, T>(a, ef, x, v, gs, hs);
-                    gv = lerp_store<T>(interior_fbar<D, T>(a, fg, f, oc, c, sn, v, xh, fcg, gcg), gs, mix);
-                    if (MODEL == CDUGKS_SHAKHOV)
-                        hv = lerp_store<T>(interior_fbar<D, T>(a, fh, f, oc, c, sn, v, xh, fch, gch), hs, mix);
+                    const bool own_c = a.cs_own[idx] != 0;
+                    const bool up_c = own_c ? up > 0 : up < 0;
+                    const bool up_o = own_c ? up < 0 : up > 0;
+                    const T *vgo = vg + (1 + k) * GS;
+                    const T *vho = vh + (1 + k) * GS;
+                    T gog[3] = {T(0), T(0), T(0)}, goh[3] = {T(0), T(0), T(0)};
+                    if (a.s.grad && __any(!up_c)) {
+                        group_grad<D, MAXN, T>(vgo, sa + (1 + k) * MAXN * 3, gog);
+                        if (SH) group_grad<D, MAXN, T>(vho, sa + (1 + k) * MAXN * 3, goh);
+                    }
+                    const T fog = vgo[0];
+                    const T Xcg = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gcg);
+                    const T Xog = dot_ext<D, T>(dn[0], dn[1], dn[2], xh, gog);
+                    // dead-band: 0.5 (f_own + f_nei + X_own + X_nei), owner first
+                    const T bg = up_c ? fcg + Xcg
+                                      : (up_o ? fog + Xog
+                                              : (own_c ? T(0.5) * (fcg + fog + Xcg + Xog)
+                                                       : T(0.5) * (fog + fcg + Xog + Xcg)));
+                    gv = lerp_store<T>(bg, gs, mix);
+                    if (SH) {
+                        const T foh = vho[0];
+                        const T Xch = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gch);
+                        const T Xoh = dot_ext<D, T>(dn[0], dn[1], dn[2], xh, goh);
+                        const T bh = up_c ? fch + Xch
+                                          : (up_o ? foh + Xoh
+                                                  : (own_c ? T(0.5) * (fch + foh + Xch + Xoh)
+                                                           : T(0.5) * (foh + fch + Xoh + Xch)));
+                        hv = lerp_store<T>(bh, hs, mix);
+                    }
                 }
-                const T knA = T(sn) * T(a.cf_sA[j]);
+                const T knA = sn * T(a.cs_sA[idx]);
                 Fg += knA * gv;
-                if (MODEL == CDUGKS_SHAKHOV) Fh += knA * hv;
+                if (SH) Fh += knA * hv;
             }
-            if constexpr (sizeof(T) == 4) {
-                // flux of f_ref itself: f_ref xi . sum(+-A n); non-zero where faces are not planar
-                const float xs = x[0] * a.cell_S[3 * c] + x[1] * a.cell_S[3 * c + 1] + x[2] * a.cell_S[3 * c + 2];
-                Fg += a.fref_f[v] * xs;
-                if (MODEL == CDUGKS_SHAKHOV) Fh += a.fref_f[a.s.nv + v] * xs;
-            }
-            const double X0 = a.xi[3 * v], X1 = a.xi[3 * v + 1], X2 = a.xi[3 * v + 2];
-            const double wv = a.w[v];
-            if (MODEL == CDUGKS_SHAKHOV) {
-                T gsn, hsn, gs, hs;
-                eq_store<MODEL, D, T>(a, cs.en, x, v, gsn, hsn);
-                eq_store<MODEL, D, T>(a, cs.e, x, v, gs, hs);
-                const T cm = T(cs.cm), C = T(cs.C), Cs = T(cs.Cs);
-                T g, h, og_, oh_;
-                if constexpr (sizeof(T) == 8) {  // cdugks_shakhov.cpp:679-695 as written
-                    const T itau = T(1.0 / cs.tau), itaun = T(1.0 / cs.taun);
-                    const T gn = cm * gsn + (T(1) - cm) * fcg;
-                    const T hn = cm * hsn + (T(1) - cm) * fch;
-                    g = C * (gn + hd * (gs * itau + (gsn - gn) * itaun) - dtv * Fg);
-                    h = C * (hn + hd * (hs * itau + (hsn - hn) * itaun) - dtv * Fh);
-                    og_ = (T(1) - Cs) * g + Cs * gs;
-                    oh_ = (T(1) - Cs) * h + Cs * hs;
+        }
+        if constexpr (sizeof(T) == 4) {
+            // flux of f_ref itself: f_ref xi . sum(+-A n); non-zero where faces are not planar
+            const float xs = x[0] * a.cell_S[3 * c] + x[1] * a.cell_S[3 * c + 1] + x[2] * a.cell_S[3 * c + 2];
+            const float2 fr = a.fref_f2[v];
+            Fg += fr.x * xs;
+            if (SH) Fh += fr.y * xs;
+        }
+        const double X0 = X.x, X1 = X.y, X2 = X.z;
+        const double wv = X.w;
+        if (SH) {
+            T gsn, hsn, gs, hs;
+            if (TAB) {
+                if constexpr (sizeof(T) == 4) {
+                    shakhov_tab_f(en_f, reinterpret_cast<const float *>(tabs) + MAXF * nt, vt, x, D, a.s.K, gref, gsn,
+                                  hsn);
+                    shakhov_tab_f(e_f, reinterpret_cast<const float *>(tabs) + (MAXF + 1) * nt, vt, x, D, a.s.K, gref,
+                                  gs, hs);
                 } else {
-                    // the same update as increments of the stored deviation:
-                    // g = g_n + C [a (g_S - g_n) + a_n (g_S^n - g_n) - dt/V F],
-                    // a = h/tau, a_n = h/tau_n, C = 1/(1 + a)
-                    const T ai = T(a.s.half_dt / cs.tau), an = T(a.s.half_dt / cs.taun);
-                    const T gn = fcg + cm * (gsn - fcg);
-                    const T hn = fch + cm * (hsn - fch);
-                    g = gn + C * (ai * (gs - gn) + an * (gsn - gn) - dtv * Fg);
-                    h = hn + C * (ai * (hs - hn) + an * (hsn - hn) - dtv * Fh);
-                    og_ = g + Cs * (gs - g);
-                    oh_ = h + Cs * (hs - h);
-                }
-                const double c0 = X0 - cs.u[0], c1 = X1 - cs.u[1], c2 = X2 - cs.u[2];
-                const double t = 0.5 * wv * ((c0 * c0 + c1 * c1 + c2 * c2) * full_value<T>(a, g, 0, v) +
-                                             full_value<T>(a, h, 1, v));
-                acc[0] += t * c0;
-                acc[1] += t * c1;
-                acc[2] += t * c2;
-                og[row + v] = og_;
-                oh[row + v] = oh_;
-            } else if (MODEL == CDUGKS_BGK) {
-                T feqn, feq, dummy;
-                eq_store<MODEL, D, T>(a, cs.en, x, v, feqn, dummy);
-                eq_store<MODEL, D, T>(a, cs.e, x, v, feq, dummy);
-                const T cm = T(cs.cm), C = T(cs.C), ceq = T(cs.Cs);
-                if constexpr (sizeof(T) == 8) {  // cdugks.cpp:325-332 as written
-                    const T itau = T(1.0 / cs.tau);
-                    const T fn = cm * feqn + (T(1) - cm) * fcg;
-                    og[row + v] =
-                        (T(1) - ceq) * (C * (fn + hd * (feq * itau + (feqn - fn) * itau) - dtv * Fg)) + ceq * feq;
-                } else {
-                    const T ai = T(a.s.half_dt / cs.tau);
-                    const T fn = fcg + cm * (feqn - fcg);
-                    const T g = fn + C * (ai * (feq - fn) + ai * (feqn - fn) - dtv * Fg);
-                    og[row + v] = g + ceq * (feq - g);
+                    shakhov_tab_d(cs.en, reinterpret_cast<const double *>(tabs) + MAXF * nt, vt, x, D, a.s.K, gsn,
+                                  hsn);
+                    shakhov_tab_d(cs.e, reinterpret_cast<const double *>(tabs) + (MAXF + 1) * nt, vt, x, D, a.s.K, gs,
+                                  hs);
                 }
             } else {
-                const double dt = a.s.dt, tau = a.s.tau;
-                const T Cbt = T((dt + a.s.half_dt) / (a.s.half_dt - 2.0 * tau));
-                T feq, dummy;
-                eq_store<MODEL, D, T>(a, cs.en, x, v, feq, dummy);
-                T fnew;
-                if constexpr (sizeof(T) == 8) {  // dugks.cpp:306-309 as written
-                    const T ft = (T(1) - Cbt) * fcg + Cbt * feq;
-                    fnew = (T(4) * fcg - ft) / T(3) - dtv * Fg;
-                } else {
-                    fnew = fcg - Cbt * (feq - fcg) / T(3) - dtv * Fg;
-                }
-                og[row + v] = fnew;
-                const double wf = wv * full_value<T>(a, fnew, 0, v);
-                acc[0] += wf;
-                acc[1] += wf * X0;
-                acc[2] += wf * X1;
-                acc[3] += wf * X2;
+                cell_eq_store<MODEL, D, T>(a, cs.en, en_f, x, v, gsn, hsn);
+                cell_eq_store<MODEL, D, T>(a, cs.e, e_f, x, v, gs, hs);
             }
+            const T cm = T(cs.cm), C = T(cs.C), Cs = T(cs.Cs);
+            T g, h, og_, oh_;
+            if constexpr (sizeof(T) == 8) {  // cdugks_shakhov.cpp:679-695 as written
+                const T itau = T(1.0 / cs.tau), itaun = T(1.0 / cs.taun);
+                const T gn = cm * gsn + (T(1) - cm) * fcg;
+                const T hn = cm * hsn + (T(1) - cm) * fch;
+                g = C * (gn + hd * (gs * itau + (gsn - gn) * itaun) - dtv * Fg);
+                h = C * (hn + hd * (hs * itau + (hsn - hn) * itaun) - dtv * Fh);
+                og_ = (T(1) - Cs) * g + Cs * gs;
+                oh_ = (T(1) - Cs) * h + Cs * hs;
+            } else {
+                // the same update as increments of the stored deviation:
+                // g = g_n + C [a (g_S - g_n) + a_n (g_S^n - g_n) - dt/V F],
+                // a = h/tau, a_n = h/tau_n, C = 1/(1 + a)
+                const T ai = T(a.s.half_dt / cs.tau), an = T(a.s.half_dt / cs.taun);
+                const T gn = fcg + cm * (gsn - fcg);
+                const T hn = fch + cm * (hsn - fch);
+                g = gn + C * (ai * (gs - gn) + an * (gsn - gn) - dtv * Fg);
+                h = hn + C * (ai * (hs - hn) + an * (hsn - hn) - dtv * Fh);
+                og_ = g + Cs * (gs - g);
+                oh_ = h + Cs * (hs - h);
+            }
+            double gfull = double(g), hfull = double(h);
+            if constexpr (sizeof(T) == 4) {
+                const double2 fr = a.fref_d2[v];
+                gfull += fr.x;
+                hfull += fr.y;
+            }
+            const double c0 = X0 - cs.u[0], c1 = X1 - cs.u[1], c2 = X2 - cs.u[2];
+            const double t = 0.5 * wv * ((c0 * c0 + c1 * c1 + c2 * c2) * gfull + hfull);
+            acc[0] += t * c0;
+            acc[1] += t * c1;
+            acc[2] += t * c2;
+            og[row + v] = og_;
+            oh[row + v] = oh_;
+        } else if (MODEL == CDUGKS_BGK) {
+            T feqn, feq, dummy;
+            cell_eq_store<MODEL, D, T>(a, cs.en, en_f, x, v, feqn, dummy);
+            cell_eq_store<MODEL, D, T>(a, cs.e, e_f, x, v, feq, dummy);
+            const T cm = T(cs.cm), C = T(cs.C), ceq = T(cs.Cs);
+            if constexpr (sizeof(T) == 8) {  // cdugks.cpp:325-332 as written
+                const T itau = T(1.0 / cs.tau);
+                const T fn = cm * feqn + (T(1) - cm) * fcg;
+                og[row + v] = (T(1) - ceq) * (C * (fn + hd * (feq * itau + (feqn - fn) * itau) - dtv * Fg)) + ceq * feq;
+            } else {
+                const T ai = T(a.s.half_dt / cs.tau);
+                const T fn = fcg + cm * (feqn - fcg);
+                const T g = fn + C * (ai * (feq - fn) + ai * (feqn - fn) - dtv * Fg);
+                og[row + v] = g + ceq * (feq - g);
+            }
+        } else {
+            const double dt = a.s.dt, tau = a.s.tau;
+            const T Cbt = T((dt + a.s.half_dt) / (a.s.half_dt - 2.0 * tau));
+            T feq, dummy;
+            cell_eq_store<MODEL, D, T>(a, cs.en, en_f, x, v, feq, dummy);
+            T fnew;
+            if constexpr (sizeof(T) == 8) {  // dugks.cpp:306-309 as written
+                const T ft = (T(1) - Cbt) * fcg + Cbt * feq;
+                fnew = (T(4) * fcg - ft) / T(3) - dtv * Fg;
+            } else {
+                fnew = fcg - Cbt * (feq - fcg) / T(3) - dtv * Fg;
+            }
+            og[row + v] = fnew;
+            const double wf = wv * full_value<T>(a, fnew, 0, v);
+            acc[0] += wf;
+            acc[1] += wf * X0;
+            acc[2] += wf * X1;
+            acc[3] += wf * X2;
         }
     }
     if (MODEL == CDUGKS_BGK) return;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc[k] = seg_sum<LP>(acc[k]);
-    if (act && li == 0) {
+    for (int k = 0; k < 4; ++k) acc[k] = seg_sum<64>(acc[k]);
+    if (lane == 0) {
         double *dst = a.part_c + (size_t(c) * a.s.ntile_c + tile) * 4;
         for (int k = 0; k < 4; ++k) dst[k] = acc[k];
     }
@@ -973,6 +1271,23 @@ struct meso_hip_ctx {
     int2 *face_cells = nullptr;
     double *face_n = nullptr, *face_d = nullptr, *cf_sA = nullptr, *nb_a = nullptr, *cell_vol = nullptr;
     float *face_df = nullptr, *nb_af = nullptr, *xif = nullptr, *cell_S = nullptr;
+    int *cs_nf = nullptr, *cs_face = nullptr, *cs_other = nullptr, *cs_own = nullptr, *cs_bnd = nullptr,
+        *lsq_cnt = nullptr, *lsq_nb = nullptr;
+    double *cs_sA = nullptr, *cs_d = nullptr, *lsq_a = nullptr;
+    float *cs_df = nullptr, *lsq_af = nullptr, *face_nf = nullptr;
+    EqF *face_eqf = nullptr;
+    double4 *vel_d = nullptr;
+    float4 *vel_f = nullptr;
+    double2 *fref_d2 = nullptr;
+    float2 *fref_f2 = nullptr;
+    int nt = 0;
+    int *vtab = nullptr, *tab_ax = nullptr;
+    double *tab_x = nullptr;
+    void *face_tab = nullptr;
+    int *cell_list = nullptr, *st_id = nullptr, *fs_id = nullptr;
+    double *st_a = nullptr, *fs_a = nullptr;
+    float *st_af = nullptr, *fs_af = nullptr;
+    int n_int = 0, n_bcell = 0;
     int *cf_ptr = nullptr, *cf_ids = nullptr, *nb_ptr = nullptr, *nb_ids = nullptr, *face_bnd = nullptr,
         *bnd_face = nullptr, *bnd_mark = nullptr, *mirror = nullptr;
     meso_mark_desc *marks = nullptr;
@@ -1080,6 +1395,36 @@ KArgs make_args(meso_hip_ctx *c) {
     a.nb_af = c->nb_af;
     a.cell_vol = c->cell_vol;
     a.cell_S = c->cell_S;
+    a.cs_nf = c->cs_nf;
+    a.cs_face = c->cs_face;
+    a.cs_other = c->cs_other;
+    a.cs_own = c->cs_own;
+    a.cs_bnd = c->cs_bnd;
+    a.cs_sA = c->cs_sA;
+    a.cs_d = c->cs_d;
+    a.cs_df = c->cs_df;
+    a.lsq_cnt = c->lsq_cnt;
+    a.lsq_nb = c->lsq_nb;
+    a.lsq_a = c->lsq_a;
+    a.lsq_af = c->lsq_af;
+    a.face_nf = c->face_nf;
+    a.face_eqf = c->face_eqf;
+    a.vel_d = c->vel_d;
+    a.vel_f = c->vel_f;
+    a.fref_d2 = c->fref_d2;
+    a.fref_f2 = c->fref_f2;
+    a.nt = c->nt;
+    a.vtab = c->vtab;
+    a.tab_x = c->tab_x;
+    a.tab_ax = c->tab_ax;
+    a.face_tab = c->face_tab;
+    a.cell_list = c->cell_list;
+    a.st_id = c->st_id;
+    a.st_a = c->st_a;
+    a.st_af = c->st_af;
+    a.fs_id = c->fs_id;
+    a.fs_a = c->fs_a;
+    a.fs_af = c->fs_af;
     a.face_bnd = c->face_bnd;
     a.bnd_face = c->bnd_face;
     a.bnd_mark = c->bnd_mark;
@@ -1168,6 +1513,14 @@ int set_reference(meso_hip_ctx *c, double rho_ref, double T_ref) {
     }
     HIP_TRY(c, hipMemcpyAsync(c->fref_d, fr.data(), fr.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->fref_f, ff.data(), ff.size() * 4, hipMemcpyHostToDevice, c->stream));
+    std::vector<double2> fr2(nv);
+    std::vector<float2> ff2(nv);
+    for (int v = 0; v < nv; ++v) {
+        fr2[v] = make_double2(fr[v], fr[nv + v]);
+        ff2[v] = make_float2(ff[v], ff[nv + v]);
+    }
+    HIP_TRY(c, hipMemcpyAsync(c->fref_d2, fr2.data(), nv * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->fref_f2, ff2.data(), nv * sizeof(float2), hipMemcpyHostToDevice, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MESO_OK;
 }
@@ -1211,24 +1564,56 @@ struct Launch {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), 0, (ctx)->stream, __VA_ARGS__);            \
     } while (0)
 
-template <int MODEL, int D, class T, int LP>
+#define LAUNCH_SH(ctx, name, kern, grid, shmem, ...)                                                 \
+    do {                                                                                             \
+        Launch l_(ctx, name);                                                                        \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), shmem, (ctx)->stream, __VA_ARGS__);        \
+    } while (0)
+
+template <int MODEL, int D, class T, bool TAB>
 int step_impl(meso_hip_ctx *c, KArgs a) {
     const StepConst &s = c->s;
-    constexpr int IPB = BLOCK / LP;
-    const int fblk = (s.nf + IPB - 1) / IPB, cblk = (s.nc + IPB - 1) / IPB;
-    LAUNCH(c, "face_moments", (k_face_moments<MODEL == CDUGKS_SHAKHOV ? CDUGKS_SHAKHOV : CDUGKS_BGK, D, T, LP>),
-           fblk * s.ntile_f, a);
-    LAUNCH(c, "face_eq", (k_face_eq<MODEL == CDUGKS_SHAKHOV ? CDUGKS_SHAKHOV : CDUGKS_BGK, D, T, LP>), fblk, a);
-    if (s.nb > 0)
-        LAUNCH(c, "boundary", (k_boundary<MODEL == CDUGKS_SHAKHOV ? CDUGKS_SHAKHOV : CDUGKS_BGK, D, T>), s.nb, a);
-    LAUNCH(c, "cell_update", (k_cell_update<MODEL, D, T, LP>), cblk * s.ntile_c, a);
+    constexpr int WPB = BLOCK / 64;
+    constexpr int MF = MODEL == CDUGKS_SHAKHOV ? CDUGKS_SHAKHOV : CDUGKS_BGK;
+    constexpr int MAXF = D == 2 ? 4 : 6;
+    const int fwblk = (s.nf + WPB - 1) / WPB;
+    LAUNCH_SH(c, "face_moments", (k_face_moments<MF, D, T>), fwblk * s.ntile_f, 0, a);
+    LAUNCH_SH(c, "face_eq", (k_face_eq<MF, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
+    if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MF, D, T>), s.nb, 0, a);
+    const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
+    if (c->n_int > 0)
+        LAUNCH_SH(c, "cell_update", (k_cell_update<MODEL, D, T, TAB, false>),
+                  ((c->n_int + WPB - 1) / WPB) * s.ntile_c, lds, a, 0, c->n_int);
+    if (c->n_bcell > 0)
+        LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, D, T, TAB, true>),
+                  ((c->n_bcell + WPB - 1) / WPB) * s.ntile_c, lds, a, c->n_int, c->n_bcell);
     if (MODEL == CDUGKS_SHAKHOV) LAUNCH(c, "cell_finish", (k_cell_finish<MODEL>), (s.nc + BLOCK - 1) / BLOCK, a);
     if (MODEL == DUGKS_BGK) {
         LAUNCH(c, "cell_finish", (k_cell_finish<MODEL>), (s.nc + BLOCK - 1) / BLOCK, a);
-        LAUNCH(c, "dugks_tobp", (k_dugks_tobp<D, T, LP>), cblk, a);
+        LAUNCH(c, "dugks_tobp", (k_dugks_tobp<D, T, 64>), (s.nc + WPB - 1) / WPB, a);
     }
     HIP_TRY(c, hipGetLastError());
     return MESO_OK;
+}
+
+int step_dispatch(meso_hip_ctx *c, KArgs a) {
+    const int m = c->s.model, D = c->s.D;
+    const bool f32 = c->storage == MESO_STORAGE_FP32;
+    const bool tab = c->nt > 0;
+#define MESO_S(M, DD)                                                                                \
+    if (m == M && D == DD) {                                                                         \
+        if (M == CDUGKS_SHAKHOV && tab)                                                              \
+            return f32 ? step_impl<M, DD, float, true>(c, a) : step_impl<M, DD, double, true>(c, a);  \
+        return f32 ? step_impl<M, DD, float, false>(c, a) : step_impl<M, DD, double, false>(c, a);   \
+    }
+    MESO_S(CDUGKS_SHAKHOV, 2)
+    MESO_S(CDUGKS_SHAKHOV, 3)
+    MESO_S(CDUGKS_BGK, 2)
+    MESO_S(CDUGKS_BGK, 3)
+    MESO_S(DUGKS_BGK, 2)
+    MESO_S(DUGKS_BGK, 3)
+#undef MESO_S
+    return set_err(c, MESO_E_INVALID, "unsupported model/dimension combination");
 }
 
 template <int MODEL, int D, class T, int LP>
@@ -1261,10 +1646,6 @@ int dispatch(meso_hip_ctx *c, Args... args) {
     return set_err(c, MESO_E_INVALID, "unsupported model/dimension/storage combination");
 }
 
-template <int M, int D, class T, int L>
-struct StepF {
-    static int run(meso_hip_ctx *c, KArgs a) { return step_impl<M, D, T, L>(c, a); }
-};
 template <int M, int D, class T, int L>
 struct InitF {
     static int run(meso_hip_ctx *c, KArgs a, const double *rp, const double *up, const double *Tp, bool fm) {
@@ -1352,8 +1733,8 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     c->s.nvp = nv >= 64 ? ((nv + 63) / 64) * 64 : nv;
     c->lp = lanes_for(nv);
     const int target = env_int("MESO_HIP_VTILE", 1024);
-    tiles_for(nv, c->lp, target, c->s.ntile_f, c->s.vt_f);
-    tiles_for(nv, c->lp, target, c->s.ntile_c, c->s.vt_c);
+    tiles_for(nv, 64, target, c->s.ntile_f, c->s.vt_f);
+    tiles_for(nv, 64, target, c->s.ntile_c, c->s.vt_c);
     // ---- validate and repack mesh arrays
     for (int f = 0; f < nf; ++f) {
         int o = m->face_cells[2 * f], n = m->face_cells[2 * f + 1];
@@ -1430,6 +1811,150 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         for (int k = 0; k < 3 * nv; ++k)
             if (d->mirror[k] < 0 || d->mirror[k] >= nv)
                 return fail(set_err(nullptr, MESO_E_INVALID, "mirror index out of range"));
+    // per-cell stencil tables (faces, cells across them, LSQ neighbours), padded
+    const int MAXF = D == 2 ? 4 : 6;
+    std::vector<int> t_nf(nc), t_face(size_t(nc) * MAXF, -1), t_other(size_t(nc) * MAXF, -1),
+        t_own(size_t(nc) * MAXF, 0), t_bnd(size_t(nc) * MAXF, -1), t_lcnt(nc), t_lnb(size_t(nc) * MAXF, 0);
+    std::vector<double> t_sA(size_t(nc) * MAXF, 0.0), t_d(size_t(nc) * MAXF * 6, 0.0),
+        t_la(size_t(nc) * MAXF * 3, 0.0);
+    std::vector<float> t_df(size_t(nc) * MAXF * 6, 0.f), t_laf(size_t(nc) * MAXF * 3, 0.f), t_nf32(3 * size_t(nf));
+    for (int cc = 0; cc < nc; ++cc) {
+        const int k0 = m->cell_face_ptr[cc], nk = m->cell_face_ptr[cc + 1] - k0;
+        const int j0 = m->nbr_ptr[cc], nj = m->nbr_ptr[cc + 1] - j0;
+        if (nk > MAXF || nj > MAXF)
+            return fail(set_err(nullptr, MESO_E_INVALID, "cell with more faces/neighbours than its kind allows"));
+        t_nf[cc] = nk;
+        t_lcnt[cc] = nj;
+        for (int k = 0; k < nk; ++k) {
+            const int f = m->cell_face_ids[k0 + k];
+            const int o = m->face_cells[2 * f], n = m->face_cells[2 * f + 1];
+            const size_t idx = size_t(cc) * MAXF + k;
+            t_face[idx] = f;
+            t_own[idx] = (o == cc) ? 1 : 0;
+            t_other[idx] = (o == n) ? -1 : (o == cc ? n : o);
+            t_bnd[idx] = c->face_bnd_h[f];
+            t_sA[idx] = sA[k0 + k];
+            const int oth = t_other[idx] >= 0 ? t_other[idx] : cc;
+            for (int r = 0; r < 3; ++r) {
+                const double ds = m->face_pos[3 * f + r] - m->cell_pos[3 * size_t(cc) + r];
+                const double dn = m->face_pos[3 * f + r] - m->cell_pos[3 * size_t(oth) + r];
+                t_d[6 * idx + r] = ds;
+                t_d[6 * idx + 3 + r] = dn;
+                t_df[6 * idx + r] = float(ds);
+                t_df[6 * idx + 3 + r] = float(dn);
+            }
+        }
+        for (int j = 0; j < nj; ++j) {
+            const size_t idx = size_t(cc) * MAXF + j;
+            t_lnb[idx] = m->nbr_ids[j0 + j];
+            for (int r = 0; r < 3; ++r) {
+                t_la[3 * idx + r] = na[3 * size_t(j0 + j) + r];
+                t_laf[3 * idx + r] = float(na[3 * size_t(j0 + j) + r]);
+            }
+        }
+    }
+    for (size_t k = 0; k < 3 * size_t(nf); ++k) t_nf32[k] = float(m->face_normal[k]);
+    // gather stencils: group 0 = the cell, group 1+k = the cell across face k
+    // (the cell itself again on a boundary face, with zero weights)
+    const int GS = 1 + MAXF, NG = 1 + MAXF;
+    std::vector<int> t_sid(size_t(nc) * NG * GS), t_fsid(size_t(nf) * 2 * GS);
+    std::vector<double> t_sa(size_t(nc) * NG * MAXF * 3, 0.0), t_fsa(size_t(nf) * 2 * MAXF * 3, 0.0);
+    std::vector<float> t_saf(t_sa.size(), 0.f), t_fsaf(t_fsa.size(), 0.f);
+    auto fill_group = [&](int cell, int *ids, double *ad) {
+        ids[0] = cell;
+        const int j0 = m->nbr_ptr[cell], nj = m->nbr_ptr[cell + 1] - j0;
+        for (int j = 0; j < MAXF; ++j) {
+            ids[1 + j] = j < nj ? m->nbr_ids[j0 + j] : cell;
+            for (int r = 0; r < 3; ++r) ad[3 * j + r] = j < nj ? na[3 * size_t(j0 + j) + r] : 0.0;
+        }
+    };
+    std::vector<int> int_cells, bnd_cells;
+    for (int cc = 0; cc < nc; ++cc) {
+        int *ids = &t_sid[size_t(cc) * NG * GS];
+        double *ad = &t_sa[size_t(cc) * NG * MAXF * 3];
+        fill_group(cc, ids, ad);
+        bool has_bnd = false;
+        for (int k = 0; k < MAXF; ++k) {
+            const size_t idx = size_t(cc) * MAXF + k;
+            const int o = (k < t_nf[cc]) ? t_other[idx] : -1;
+            if (k < t_nf[cc] && o < 0) has_bnd = true;
+            fill_group(o >= 0 ? o : cc, ids + (1 + k) * GS, ad + (1 + k) * MAXF * 3);
+            if (o < 0)
+                for (int j = 0; j < MAXF * 3; ++j) ad[(1 + k) * MAXF * 3 + j] = 0.0;
+        }
+        (has_bnd ? bnd_cells : int_cells).push_back(cc);
+    }
+    for (int f = 0; f < nf; ++f) {
+        fill_group(m->face_cells[2 * f], &t_fsid[size_t(f) * 2 * GS], &t_fsa[size_t(f) * 2 * MAXF * 3]);
+        fill_group(m->face_cells[2 * f + 1], &t_fsid[size_t(f) * 2 * GS + GS],
+                   &t_fsa[size_t(f) * 2 * MAXF * 3 + MAXF * 3]);
+    }
+    for (size_t k = 0; k < t_sa.size(); ++k) t_saf[k] = float(t_sa[k]);
+    for (size_t k = 0; k < t_fsa.size(); ++k) t_fsaf[k] = float(t_fsa[k]);
+    c->n_int = int(int_cells.size());
+    c->n_bcell = int(bnd_cells.size());
+    std::vector<int> t_list(int_cells);
+    t_list.insert(t_list.end(), bnd_cells.begin(), bnd_cells.end());
+    // velocity slots and, for a tensor-product set, the axis tables
+    std::vector<double4> t_vd(nv);
+    std::vector<float4> t_vf(nv);
+    for (int v = 0; v < nv; ++v) {
+        const double *x = d->xi + 3 * size_t(v);
+        t_vd[v] = make_double4(x[0], x[1], x[2], d->w[v]);
+        t_vf[v] = make_float4(float(x[0]), float(x[1]), float(x[2]), float(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]));
+    }
+    std::vector<int> t_vtab, t_tax;
+    std::vector<double> t_tx;
+    if (p->model == MESO_MODEL_CDUGKS_SHAKHOV && env_int("MESO_HIP_NO_TABLES", 0) == 0) {
+        // unique abscissae per axis; tensor when every (i,j,k) combination occurs once
+        std::vector<std::vector<double>> axes(D);
+        for (int dd = 0; dd < D; ++dd) {
+            for (int v = 0; v < nv; ++v) axes[dd].push_back(d->xi[3 * size_t(v) + dd]);
+            std::sort(axes[dd].begin(), axes[dd].end());
+            axes[dd].erase(std::unique(axes[dd].begin(), axes[dd].end()), axes[dd].end());
+        }
+        size_t prod = 1, tot = 0;
+        for (int dd = 0; dd < D; ++dd) {
+            prod *= axes[dd].size();
+            tot += axes[dd].size();
+        }
+        bool third_zero = true;
+        if (D == 2)
+            for (int v = 0; v < nv; ++v) third_zero = third_zero && d->xi[3 * size_t(v) + 2] == 0.0;
+        if (prod == size_t(nv) && tot < 1024 && third_zero) {
+            std::vector<int> seen(prod, 0);
+            bool ok = true;
+            t_vtab.resize(nv);
+            int off[3] = {0, 0, 0};
+            for (int dd = 1; dd < D; ++dd) off[dd] = off[dd - 1] + int(axes[dd - 1].size());
+            for (int v = 0; v < nv && ok; ++v) {
+                int idx[3] = {0, 0, 0};
+                size_t lin = 0;
+                for (int dd = D - 1; dd >= 0; --dd) {
+                    const double xv = d->xi[3 * size_t(v) + dd];
+                    idx[dd] = int(std::lower_bound(axes[dd].begin(), axes[dd].end(), xv) - axes[dd].begin());
+                    lin = lin * axes[dd].size() + idx[dd];
+                }
+                if (seen[lin]++) ok = false;
+                int packed = 0;
+                for (int dd = 0; dd < D; ++dd) packed |= (off[dd] + idx[dd]) << (10 * dd);
+                t_vtab[v] = packed;
+            }
+            if (ok) {
+                for (int dd = 0; dd < D; ++dd)
+                    for (double xv : axes[dd]) {
+                        t_tx.push_back(xv);
+                        t_tax.push_back(dd);
+                    }
+                c->nt = int(t_tx.size());
+            }
+        }
+    }
+    {
+        // the cell kernel keeps MAXF + 2 tables per wave in LDS: stay within 64 KiB
+        const size_t lds = size_t(BLOCK / 64) * (MAXF + 2) * size_t(c->nt) * (storage == MESO_STORAGE_FP32 ? 4 : 8);
+        if (lds > 65536) c->nt = 0;
+    }
     int rc = 0;
 #define UP(dst, src, n) \
     if ((rc = upload(c, &c->dst, src, n))) return fail(rc);
@@ -1446,6 +1971,33 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     UP(nb_af, naf.data(), naf.size());
     UP(cell_vol, m->cell_vol, nc);
     UP(cell_S, cS.data(), cS.size());
+    UP(cs_nf, t_nf.data(), t_nf.size());
+    UP(cs_face, t_face.data(), t_face.size());
+    UP(cs_other, t_other.data(), t_other.size());
+    UP(cs_own, t_own.data(), t_own.size());
+    UP(cs_bnd, t_bnd.data(), t_bnd.size());
+    UP(cs_sA, t_sA.data(), t_sA.size());
+    UP(cs_d, t_d.data(), t_d.size());
+    UP(cs_df, t_df.data(), t_df.size());
+    UP(lsq_cnt, t_lcnt.data(), t_lcnt.size());
+    UP(lsq_nb, t_lnb.data(), t_lnb.size());
+    UP(lsq_a, t_la.data(), t_la.size());
+    UP(lsq_af, t_laf.data(), t_laf.size());
+    UP(face_nf, t_nf32.data(), t_nf32.size());
+    UP(cell_list, t_list.data(), t_list.size());
+    UP(st_id, t_sid.data(), t_sid.size());
+    UP(st_a, t_sa.data(), t_sa.size());
+    UP(st_af, t_saf.data(), t_saf.size());
+    UP(fs_id, t_fsid.data(), t_fsid.size());
+    UP(fs_a, t_fsa.data(), t_fsa.size());
+    UP(fs_af, t_fsaf.data(), t_fsaf.size());
+    UP(vel_d, t_vd.data(), t_vd.size());
+    UP(vel_f, t_vf.data(), t_vf.size());
+    if (c->nt > 0) {
+        UP(vtab, t_vtab.data(), t_vtab.size());
+        UP(tab_x, t_tx.data(), t_tx.size());
+        UP(tab_ax, t_tax.data(), t_tax.size());
+    }
     UP(face_bnd, c->face_bnd_h.data(), nf);
     UP(bnd_face, c->bnd_face_h.data(), c->bnd_face_h.size());
     UP(bnd_mark, bmark.data(), bmark.size());
@@ -1475,10 +2027,16 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         (rc = dalloc(c, &c->face_flux, size_t(nf) * (D + 2))) ||
         (rc = dalloc(c, &c->part_c, size_t(nc) * c->s.ntile_c * 4)) || (rc = dalloc(c, &c->fm, 5 * size_t(nc))) ||
         (rc = dalloc(c, &c->res_part, 8 * size_t((nc + BLOCK - 1) / BLOCK))) || (rc = dalloc(c, &c->res_out, 8)) ||
-        (rc = dalloc(c, &c->face_eq, nf)) || (rc = dalloc(c, &c->bnd, std::max(1, c->s.nb))) ||
+        (rc = dalloc(c, &c->face_eq, nf)) || (rc = dalloc(c, &c->face_eqf, nf)) || (rc = dalloc(c, &c->bnd, std::max(1, c->s.nb))) ||
         (rc = dalloc(c, &c->flags, 4)) || (rc = dalloc(c, &c->fref_d, 2 * size_t(nv))) ||
-        (rc = dalloc(c, &c->fref_f, 2 * size_t(nv))))
+        (rc = dalloc(c, &c->fref_f, 2 * size_t(nv))) || (rc = dalloc(c, &c->fref_d2, nv)) ||
+        (rc = dalloc(c, &c->fref_f2, nv)))
         return fail(rc);
+    if (c->nt > 0) {
+        char *ft = nullptr;
+        if ((rc = dalloc(c, &ft, size_t(nf) * c->nt * es))) return fail(rc);
+        c->face_tab = ft;
+    }
     c->xi_h.assign(d->xi, d->xi + 3 * size_t(nv));
     c->w_h.assign(d->w, d->w + nv);
     if (hipHostMalloc((void **)&c->flags_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
@@ -1664,7 +2222,7 @@ int meso_hip_step(meso_hip_ctx *c, int32_t nsteps, meso_step_report *rep) {
     for (int i = 0; i < nsteps; ++i) {
         KArgs a = make_args(c);
         a.step_index = c->steps_done + i;
-        if ((rc = dispatch<StepF>(c, a))) return rc;
+        if ((rc = step_dispatch(c, a))) return rc;
         c->cur = 1 - c->cur;
     }
     if ((rc = sync_flags(c))) return rc;

@@ -46,6 +46,98 @@ struct EqParams {
     double lnAr, dB, dhk;
 };
 
+// FP32 copy of the deviation form of EqParams, converted once per face so the
+// per-velocity loop reads 16 floats: g_S - g_ref = g_ref [expm1(dl)(1 + S) + S],
+// dl = lnAr + dB |xi|^2 + B2 xi.u - Buu.
+struct EqF {
+    float u[3], q[3];
+    float lnAr, dB, B2, Buu, iRT, P1, P2, hk, dhk, mix;
+    float a0, oa0;  // expm1(lnAr), 1 + expm1(lnAr) for the axis-table evaluation
+};
+
+__host__ __device__ inline EqF to_eqf(const EqParams &e) {
+    EqF r;
+    for (int k = 0; k < 3; ++k) {
+        r.u[k] = float(e.u[k]);
+        r.q[k] = float(e.q[k]);
+    }
+    r.lnAr = float(e.lnAr);
+    r.dB = float(e.dB);
+    r.B2 = float(2.0 * e.B);
+    r.Buu = float(e.B * (e.u[0] * e.u[0] + e.u[1] * e.u[1] + e.u[2] * e.u[2]));
+    r.iRT = float(e.iRT);
+    r.P1 = float(e.P1);
+    r.P2 = float(e.P2);
+    r.hk = float(e.hk);
+    r.dhk = float(e.dhk);
+    r.mix = float(e.mix);
+    const double a0 = expm1(e.lnAr);
+    r.a0 = float(a0);
+    r.oa0 = float(1.0 + a0);
+    return r;
+}
+
+// Axis tables for tensor-product velocity sets. With xi = (X_i, Y_j, Z_k) the
+// Maxwellian factorises over axes:
+//   FP64 storage: g_M = A Tx[i] Ty[j] Tz[k],  T_d[i] = exp(-B (X_i - u_d)^2)
+//   FP32 storage: g_M / g_ref - 1 = (1 + a0) prod_d (1 + T_d[i]) - 1,
+//                 T_d[i] = expm1(B_ref X_i^2 - B (X_i - u_d)^2), a0 = expm1(lnAr)
+// so the per-velocity work is a few multiplies instead of an exponential.
+// vt packs the table offsets of a velocity's axis entries (10 bits each).
+__device__ __forceinline__ void shakhov_tab_f(const EqF &e, const float *tb, int vt, const float xi[3], int D, int K,
+                                              float gref, float &gs, float &hs) {
+    const float ax = tb[vt & 1023], ay = tb[(vt >> 10) & 1023];
+    float t = fmaf(ax, ay, ax + ay);
+    if (D == 3) {
+        const float az = tb[(vt >> 20) & 1023];
+        t = fmaf(t, az, t + az);
+    }
+    const float em1 = fmaf(t, e.oa0, e.a0);
+    const float c0 = xi[0] - e.u[0], c1 = xi[1] - e.u[1], c2 = xi[2] - e.u[2];
+    const float cc = c0 * c0 + c1 * c1 + c2 * c2;
+    const float cq = c0 * e.q[0] + c1 * e.q[1] + c2 * e.q[2];
+    const float ccr = cc * e.iRT;
+    const float S = e.P1 * cq * (ccr - float(D) - 2.0f);
+    const float H = e.P2 * cq * ((ccr - float(D)) * float(K + 3 - D) - float(2 * K));
+    gs = gref * fmaf(S, 1.0f + em1, em1);
+    hs = gref * fmaf(em1, e.hk + H, e.dhk + H);
+}
+
+__device__ __forceinline__ void shakhov_tab_d(const EqParams &e, const double *tb, int vt, const double xi[3], int D,
+                                              int K, double &gs, double &hs) {
+    double gm = e.A * tb[vt & 1023] * tb[(vt >> 10) & 1023];
+    if (D == 3) gm *= tb[(vt >> 20) & 1023];
+    const double c0 = xi[0] - e.u[0], c1 = xi[1] - e.u[1], c2 = xi[2] - e.u[2];
+    const double cc = c0 * c0 + c1 * c1 + c2 * c2;
+    const double cq = c0 * e.q[0] + c1 * e.q[1] + c2 * e.q[2];
+    const double ccr = cc * e.iRT;
+    gs = gm * (1.0 + e.P1 * cq * (ccr - double(D) - 2.0));
+    hs = gm * (e.hk + e.P2 * cq * ((ccr - double(D)) * double(K + 3 - D) - double(2 * K)));
+}
+
+__device__ __forceinline__ void shakhov_devf(const EqF &e, const float xi[3], int D, int K, float gref, float &gs,
+                                             float &hs) {
+    const float c0 = xi[0] - e.u[0], c1 = xi[1] - e.u[1], c2 = xi[2] - e.u[2];
+    const float cc = c0 * c0 + c1 * c1 + c2 * c2;
+    const float cq = c0 * e.q[0] + c1 * e.q[1] + c2 * e.q[2];
+    const float xx = xi[0] * xi[0] + xi[1] * xi[1] + xi[2] * xi[2];
+    const float xu = xi[0] * e.u[0] + xi[1] * e.u[1] + xi[2] * e.u[2];
+    const float dl = e.lnAr + e.dB * xx + e.B2 * xu - e.Buu;
+    const float em1 = expm1f(dl);
+    const float ccr = cc * e.iRT;
+    const float S = e.P1 * cq * (ccr - float(D) - 2.0f);
+    const float H = e.P2 * cq * ((ccr - float(D)) * float(K + 3 - D) - float(2 * K));
+    gs = gref * (em1 * (1.0f + S) + S);
+    hs = gref * (em1 * (e.hk + H) + e.dhk + H);
+}
+
+__device__ __forceinline__ float bgk_devf(const EqF &e, const float xi[3]) {
+    // for BGK the EqF fields hold: lnAr = rho - rho_ref, hk = rho
+    const float ku = (xi[0] * e.u[0] + xi[1] * e.u[1] + xi[2] * e.u[2]) * e.iRT;
+    const float uu = (e.u[0] * e.u[0] + e.u[1] * e.u[1] + e.u[2] * e.u[2]) * e.iRT;
+    return e.lnAr + e.hk * (ku + (ku * ku - uu) * 0.5f);
+}
+
 // Parameters of one boundary face after the wall / inlet / outlet stage: the
 // value imposed on incoming velocities is the equilibrium `in` (Maxwellian
 // of rho_b, u_b, T_b; BGK polynomial), optionally mixed with the mirror

@@ -45,6 +45,43 @@ struct KArgs {
     const double *cell_vol;
     const float *cell_S;        // [nc*3] sum over the cell's faces of +-A n (0 for a closed, planar-faced cell)
     const int *face_bnd;        // [nf] boundary index or -1
+    // per-cell stencil tables, padded to MAXF faces / MAXN neighbours
+    // (2D: 4, 3D: 6), in the cell's local face order
+    const int *cs_nf;           // [nc] faces of the cell
+    const int *cs_face;         // [nc*MAXF] face id
+    const int *cs_other;        // [nc*MAXF] cell across the face, -1 on a boundary
+    const int *cs_own;          // [nc*MAXF] 1 when the cell owns the face
+    const int *cs_bnd;          // [nc*MAXF] boundary index or -1
+    const double *cs_sA;        // [nc*MAXF] +-area
+    const double *cs_d;         // [nc*MAXF*6] x_f - x_c, x_f - x_other
+    const float *cs_df;
+    const int *lsq_cnt;         // [nc] LSQ neighbours
+    const int *lsq_nb;          // [nc*MAXN]
+    const double *lsq_a;        // [nc*MAXN*3] w_j (Cx.dr_j, Cy.dr_j, Cz.dr_j)
+    const float *lsq_af;
+    const float *face_nf;       // [nf*3] float copy of the normals
+    EqF *face_eqf;              // [nf] FP32 copy of face_eq
+    // velocity slots
+    const double4 *vel_d;       // [nv] x, y, z, w
+    const float4 *vel_f;        // [nv] x, y, z, |xi|^2
+    const double2 *fref_d2;     // [nv] g_ref, h_ref
+    const float2 *fref_f2;
+    // axis tables of tensor-product velocity sets (nt == 0: none)
+    int nt;
+    const int *vtab;            // [nv] packed offsets of a velocity's axis entries
+    const double *tab_x;        // [nt] abscissa of each entry
+    const int *tab_ax;          // [nt] axis of each entry
+    void *face_tab;             // [nf*nt] per-face axis factors (storage type)
+    // gather stencils: group = centre cell + its LSQ neighbours (padded with
+    // the centre and zero weights); cells: groups of the cell and of the cell
+    // across each face; faces: groups of owner and neighbour
+    const int *cell_list;       // interior cells, then cells with a boundary face
+    const int *st_id;           // [nc][(1+MAXF)*(1+MAXN)]
+    const double *st_a;         // [nc][(1+MAXF)*MAXN*3]
+    const float *st_af;
+    const int *fs_id;           // [nf][2*(1+MAXN)]
+    const double *fs_a;         // [nf][2*MAXN*3]
+    const float *fs_af;
     const int *bnd_face;        // [nb]
     const int *bnd_mark;        // [nb] index into marks
     const void *marks;          // meso_mark_desc[] (host layout)

@@ -53,6 +53,10 @@ def case_factory(tmp_path_factory):
             p = mg.shakhov_cavity_case(d, mg.quad_cavity(12, jitter=0.1, seed=5), ("nc", 1, 23, 4.5), mu075, 0.2)
         elif name == "shak_tri6":
             p = mg.shakhov_cavity_case(d, mg.tri_cavity(6, jitter=0.2, seed=1), ("nc", 2, 4, 4.5), mu075, 0.2)
+        elif name == "shak_tri6_cfl02":
+            # CFL 0.5 is unstable on this coarse jittered mesh after ~50 steps (the oracle diverges too)
+            p = mg.shakhov_cavity_case(d, mg.tri_cavity(6, jitter=0.2, seed=1), ("nc", 2, 4, 4.5), mu075, 0.2,
+                                       cfl=0.2)
         elif name == "shak_hex3":
             p = mg.shakhov_cavity_case(d, mg.hex_cavity(3, jitter=0.05, seed=2), ("sparse-gh", 4, 0.5),
                                        mg.viscosity_from_kn(0.1), 0.1)

@@ -59,7 +59,7 @@ def test_fp64_matches_oracle(case_factory, name):
                 assert np.allclose(a, b, rtol=1e-9, atol=1e-14), f"{name} step {s} flux moments"
 
 
-@pytest.mark.parametrize("name", ["bgk_quad8", "shak_quad6", "shak_tri6", "shak_hex3", "shak_quad12_nc24"])
+@pytest.mark.parametrize("name", ["bgk_quad8", "shak_quad6", "shak_tri6_cfl02", "shak_hex3", "shak_quad12_nc24"])
 def test_fp32_storage_close_to_oracle(case_factory, name):
     """FP32 storage of the distributions on a developed flow (200 steps)."""
     cs, dev, orc, out = run_pair(case_factory(name), [200], storage=capi.STORAGE_FP32)
