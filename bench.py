@@ -14,6 +14,7 @@ read + write of g and h in FP32) and the CPU baseline (the FP64 oracle with
 OpenMP on this host, one step of the same workload).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scale s]
+                       [--storage both|fp32|fp64] [--copy-calibration MIB]
 """
 import argparse
 import json
@@ -79,6 +80,20 @@ def cpu_baseline(case_path, steps):
                       f"FP64 oracle with OpenMP, {dt:.1f} s"}
 
 
+def copy_calibration(mib):
+    """A plain float32 device-to-device copy of `mib` MiB, three launches: the
+    HBM byte counters of one launch should read `mib` MiB fetched and written."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: 3 x {mib} MiB copies done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,7 +103,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--scale", type=float, default=1.0, help="mesh resolution scale (<1: quick runs)")
     ap.add_argument("--workload", default=WORKLOAD)
+    ap.add_argument("--storage", choices=("both", "fp32", "fp64"), default="both",
+                    help="storage kinds to run (profiling runs use one; the JSON line needs both)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the step is benchmarked on one MI355X (--gpus {args.gpus} "
               "is not supported; the multi-GPU velocity split is a design, not a measured path).", file=sys.stderr)
@@ -104,7 +126,10 @@ def main():
 
     res = {}
     macros = {}
-    for storage, tag in ((capi.STORAGE_FP32, "fp32"), (capi.STORAGE_FP64, "fp64")):
+    kinds = [(capi.STORAGE_FP32, "fp32"), (capi.STORAGE_FP64, "fp64")]
+    if args.storage != "both":
+        kinds = [k for k in kinds if k[1] == args.storage]
+    for storage, tag in kinds:
         dev = capi.Device(cs, storage)
         dev.init_equilibrium()
         secs = time_steps(dev, args.steps, args.warmup)
@@ -116,6 +141,8 @@ def main():
             log(f"          {k:14s} {v['avg_ms']:.4f} ms x{v['launches']}")
         dev.close()
 
+    if args.storage != "both":
+        return
     # precision of the FP32-storage step against the FP64 step (same steps)
     prec = {}
     a, b = macros["fp32"], macros["fp64"]

@@ -929,4 +929,12 @@ int oracle_face_values(const oracle_ctx *o, double *ff) {
     return MESO_OK;
 }
 
+int oracle_fbar(oracle_ctx *o, const double *cell_f, double *face_f) {
+    if (!o || !cell_f || !face_f) return fail(MESO_E_INVALID, "null argument");
+    std::vector<double> cv(cell_f, cell_f + size_t(o->NV) * o->NC), fv(size_t(o->NV) * o->NF);
+    o->reconstruct_faces(cv, fv);
+    std::memcpy(face_f, fv.data(), fv.size() * sizeof(double));
+    return MESO_OK;
+}
+
 }  // extern "C"

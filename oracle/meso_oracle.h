@@ -46,6 +46,10 @@ int oracle_grad(const oracle_ctx *ctx, const double *field, double *grad);
 /* Face values g_face/h_face (velocity-major [n_d][NV][NF]) produced by the
  * last step's reconstruct(), i.e. after equilibrium mixing and boundaries. */
 int oracle_face_values(const oracle_ctx *ctx, double *f_face);
+/* Half-step face reconstruction f-bar+ alone (no mixing, no boundaries) of a
+ * velocity-major field cell_f[NV][NC] into face_f[NV][NF]
+ * (cdugks_shakhov.cpp:213-299); for checks of the upwind extrapolation. */
+int oracle_fbar(oracle_ctx *ctx, const double *cell_f, double *face_f);
 
 #ifdef __cplusplus
 }

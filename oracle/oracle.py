@@ -46,6 +46,7 @@ def lib():
             "oracle_residual": ([vp, _d], C.c_int),
             "oracle_grad": ([vp, _d, _d], C.c_int),
             "oracle_face_values": ([vp, _d], C.c_int),
+            "oracle_fbar": ([vp, _d, _d], C.c_int),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -146,3 +147,10 @@ class Oracle:
         ff = np.zeros(self.nd * self.nv * self.nf)
         self._check(lib().oracle_face_values(self._h, dptr(ff)))
         return ff.reshape(self.nd, self.nv, self.nf)
+
+    def fbar(self, cell_f):
+        """f-bar+ at the faces of a velocity-major field [NV][NC] (no mixing, no boundaries)."""
+        cf = as_f64(cell_f).reshape(self.nv, self.nc)
+        ff = np.zeros(self.nv * self.nf)
+        self._check(lib().oracle_fbar(self._h, dptr(cf), dptr(ff)))
+        return ff.reshape(self.nv, self.nf)
