@@ -1251,6 +1251,8 @@ __global__ void k_reset_flags(int *bad_count, int *first_bad, int *bad_step, int
     *halt = 0;
 }
 
+#include "patch_kernels.h"
+
 }  // namespace
 
 // ====================================================================== host
@@ -1288,6 +1290,18 @@ struct meso_hip_ctx {
     double *st_a = nullptr, *fs_a = nullptr;
     float *st_af = nullptr, *fs_af = nullptr;
     int n_int = 0, n_bcell = 0;
+    // patch kernels (2D, large velocity sets)
+    bool patches = false;
+    int n_up = 0, up_maxi = 0, up_maxf = 0, up_maxo = 0, up_maxr = 0, n_mp = 0, mp_maxi = 0, mp_maxf = 0,
+        mp_maxr = 0;
+    int p_chunks = 0, p_tpc = 0, p_vt = 0;
+    size_t lds_up = 0, lds_mp = 0;
+    meso_patch::Item *up_items = nullptr, *mp_items = nullptr;
+    meso_patch::Face *up_faces = nullptr, *mp_faces = nullptr;
+    int *up_item_ptr = nullptr, *up_face_ptr = nullptr, *up_nown = nullptr, *mp_item_ptr = nullptr,
+        *mp_face_ptr = nullptr, *up_row_ptr = nullptr, *up_rows = nullptr, *mp_row_ptr = nullptr,
+        *mp_rows = nullptr;
+    void *crec = nullptr, *cell_tab = nullptr;
     int *cf_ptr = nullptr, *cf_ids = nullptr, *nb_ptr = nullptr, *nb_ids = nullptr, *face_bnd = nullptr,
         *bnd_face = nullptr, *bnd_mark = nullptr, *mirror = nullptr;
     meso_mark_desc *marks = nullptr;
@@ -1456,6 +1470,33 @@ KArgs make_args(meso_hip_ctx *c) {
     a.first_bad = c->flags + 1;
     a.bad_step = c->flags + 2;
     a.halt = c->flags + 3;
+    a.n_up = c->n_up;
+    a.up_maxi = c->up_maxi;
+    a.up_maxf = c->up_maxf;
+    a.up_maxo = c->up_maxo;
+    a.up_items = c->up_items;
+    a.up_faces = c->up_faces;
+    a.up_item_ptr = c->up_item_ptr;
+    a.up_face_ptr = c->up_face_ptr;
+    a.up_nown = c->up_nown;
+    a.up_maxr = c->up_maxr;
+    a.up_row_ptr = c->up_row_ptr;
+    a.up_rows = c->up_rows;
+    a.mp_maxr = c->mp_maxr;
+    a.mp_row_ptr = c->mp_row_ptr;
+    a.mp_rows = c->mp_rows;
+    a.n_mp = c->n_mp;
+    a.mp_maxi = c->mp_maxi;
+    a.mp_maxf = c->mp_maxf;
+    a.mp_items = c->mp_items;
+    a.mp_faces = c->mp_faces;
+    a.mp_item_ptr = c->mp_item_ptr;
+    a.mp_face_ptr = c->mp_face_ptr;
+    a.p_chunks = c->p_chunks;
+    a.p_tpc = c->p_tpc;
+    a.n_int_cells = c->n_int;
+    a.crec = c->crec;
+    a.cell_tab = c->cell_tab;
     a.ref = c->ref;
     a.fref_d = c->fref_d;
     a.fref_f = c->fref_f;
@@ -1545,14 +1586,14 @@ struct Launch {
     hipEvent_t a = nullptr, b = nullptr;
     Launch(meso_hip_ctx *c_, const char *n) : c(c_), name(n) {
         if (c->profiling) {
-            hipEventCreate(&a);
-            hipEventCreate(&b);
-            hipEventRecord(a, c->stream);
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, c->stream);
         }
     }
     ~Launch() {
         if (c->profiling) {
-            hipEventRecord(b, c->stream);
+            (void)hipEventRecord(b, c->stream);
             c->recs.push_back({name, a, b});
         }
     }
@@ -1570,12 +1611,46 @@ struct Launch {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCK), shmem, (ctx)->stream, __VA_ARGS__);        \
     } while (0)
 
+#define LAUNCH_P(ctx, name, kern, grid, shmem, ...)                                                  \
+    do {                                                                                             \
+        Launch l_(ctx, name);                                                                        \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCKP), shmem, (ctx)->stream, __VA_ARGS__);       \
+    } while (0)
+
+// 2D step with the LDS-staged patch kernels for the face moments and the
+// interior cells; boundary faces and boundary cells keep their kernels.
+template <int MODEL, class T, bool TAB>
+int step_patch(meso_hip_ctx *c, KArgs a) {
+    const StepConst &s = c->s;
+    constexpr int WPB = BLOCK / 64;
+    constexpr int VT = sizeof(T) == 4 ? 32 : 16;
+    constexpr int MAXF = 4;
+    const int fwblk = (s.nf + WPB - 1) / WPB;
+    LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VT>), c->n_mp * c->p_chunks, c->lds_mp, a);
+    LAUNCH_SH(c, "face_eq", (k_face_eq<MODEL, 2, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
+    if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, 2, T>), s.nb, 0, a);
+    if (c->n_int > 0) {
+        LAUNCH(c, "cell_prep", (k_cell_prep<MODEL, T, TAB>), (c->n_int + WPB - 1) / WPB, a);
+        LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VT, TAB>), c->n_up * c->p_chunks, c->lds_up, a);
+    }
+    const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
+    if (c->n_bcell > 0)
+        LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, 2, T, TAB, true>),
+                  ((c->n_bcell + WPB - 1) / WPB) * s.ntile_c, lds, a, c->n_int, c->n_bcell);
+    if (MODEL == CDUGKS_SHAKHOV) LAUNCH(c, "cell_finish", (k_cell_finish<MODEL>), (s.nc + BLOCK - 1) / BLOCK, a);
+    HIP_TRY(c, hipGetLastError());
+    return MESO_OK;
+}
+
 template <int MODEL, int D, class T, bool TAB>
 int step_impl(meso_hip_ctx *c, KArgs a) {
     const StepConst &s = c->s;
     constexpr int WPB = BLOCK / 64;
     constexpr int MF = MODEL == CDUGKS_SHAKHOV ? CDUGKS_SHAKHOV : CDUGKS_BGK;
     constexpr int MAXF = D == 2 ? 4 : 6;
+    if constexpr (D == 2 && MODEL != DUGKS_BGK) {
+        if (c->patches) return step_patch<MODEL, T, TAB>(c, a);
+    }
     const int fwblk = (s.nf + WPB - 1) / WPB;
     LAUNCH_SH(c, "face_moments", (k_face_moments<MF, D, T>), fwblk * s.ntile_f, 0, a);
     LAUNCH_SH(c, "face_eq", (k_face_eq<MF, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
@@ -1720,7 +1795,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         meso_hip_destroy(c);
         return rc;
     };
-    hipGetDevice(&c->device);
+    (void)hipGetDevice(&c->device);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(nullptr, MESO_E_DEVICE, "hipStreamCreate failed"));
     c->storage = storage;
@@ -1950,6 +2025,70 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             }
         }
     }
+    // ---- LDS-staged patch kernels: 2D CDUGKS models with large velocity sets
+    meso_patch::PatchSet ups, mps;
+    if (D == 2 && p->model != MESO_MODEL_DUGKS_BGK && nv >= 64 && env_int("MESO_HIP_PATCH", 1) != 0) {
+        const int VT = storage == MESO_STORAGE_FP32 ? 32 : 16;
+        const size_t es = storage == MESO_STORAGE_FP32 ? 4 : 8;
+        const int ND = c->nd;
+        const int ncol = c->nd == 2 ? 11 : 6;
+        constexpr int NWP = BLOCKP / 64;
+        const size_t cap = 65536;
+        meso_patch::MeshView mv;
+        mv.nc = nc;
+        mv.nf = nf;
+        mv.face_cells = m->face_cells;
+        mv.cell_pos = m->cell_pos;
+        mv.face_pos = m->face_pos;
+        mv.face_area = m->face_area;
+        mv.face_normal = m->face_normal;
+        mv.cf_ptr = m->cell_face_ptr;
+        mv.cf_ids = m->cell_face_ids;
+        mv.nb_ptr = m->nbr_ptr;
+        mv.nb_ids = m->nbr_ids;
+        mv.nb_a = na.data();
+        auto r16 = [](size_t x) { return (x + 15) & ~size_t(15); };
+        std::string perr;
+        bool ok = true;
+        const size_t si = sizeof(meso_patch::Item), sf = sizeof(meso_patch::Face);
+        for (int P = env_int("MESO_HIP_PATCH_CELLS", 32); ok; P = P * 3 / 4) {
+            ok = meso_patch::build_update_patches(mv, P, ups, perr);
+            // items | faces | rows | values | face fluxes | q partials (k_patch_update)
+            c->lds_up = r16(ups.max_items * si + ups.max_faces * sf + 4 * size_t(ups.max_rows)) +
+                        r16(size_t(ND) * ups.max_rows * VT * es + size_t(ND) * ups.max_faces * VT * es) +
+                        size_t(ups.max_owned) * 16;
+            if (c->lds_up <= cap || P <= 4) break;
+        }
+        for (int F = env_int("MESO_HIP_PATCH_FACES", 64); ok; F = F * 3 / 4) {
+            ok = meso_patch::build_moment_patches(mv, F, mps, perr);
+            const int nfg = (mps.max_faces + 63) / 64;
+            // items | faces | rows | tile velocities | values | f-bar+ rows (k_patch_moments)
+            const size_t a1 = ((mps.max_items * si + mps.max_faces * sf + 4 * size_t(mps.max_rows) + 31) & ~size_t(31)) +
+                              size_t(VT) * 32 + size_t(ND) * mps.max_rows * VT * es +
+                              size_t(ND) * mps.max_faces * (VT + 1) * es;
+            const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer, any face-group count
+            c->lds_mp = std::max(a1, a2);
+            if ((c->lds_mp <= cap && nfg <= NWP) || F <= 4) break;
+        }
+        if (ok && c->lds_up <= cap && c->lds_mp <= cap && ups.n_patch > 0 && mps.n_patch > 0) {
+            c->patches = true;
+            c->p_vt = VT;
+            const int ntiles = (nv + VT - 1) / VT;
+            c->p_tpc = std::max(1, (env_int("MESO_HIP_PCHUNK", 2560) + VT - 1) / VT);
+            c->p_chunks = (ntiles + c->p_tpc - 1) / c->p_tpc;
+            c->s.ntile_c = c->s.ntile_f = c->p_chunks;
+            c->s.vt_c = c->s.vt_f = c->p_tpc * VT;
+            c->n_up = ups.n_patch;
+            c->up_maxi = ups.max_items;
+            c->up_maxf = ups.max_faces;
+            c->up_maxo = ups.max_owned;
+            c->up_maxr = ups.max_rows;
+            c->n_mp = mps.n_patch;
+            c->mp_maxi = mps.max_items;
+            c->mp_maxf = mps.max_faces;
+            c->mp_maxr = mps.max_rows;
+        }
+    }
     {
         // the cell kernel keeps MAXF + 2 tables per wave in LDS: stay within 64 KiB
         const size_t lds = size_t(BLOCK / 64) * (MAXF + 2) * size_t(c->nt) * (storage == MESO_STORAGE_FP32 ? 4 : 8);
@@ -1991,6 +2130,21 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     UP(fs_id, t_fsid.data(), t_fsid.size());
     UP(fs_a, t_fsa.data(), t_fsa.size());
     UP(fs_af, t_fsaf.data(), t_fsaf.size());
+    if (c->patches) {
+        UP(up_items, ups.items.data(), ups.items.size());
+        UP(up_faces, ups.faces.data(), ups.faces.size());
+        UP(up_item_ptr, ups.item_ptr.data(), ups.item_ptr.size());
+        UP(up_face_ptr, ups.face_ptr.data(), ups.face_ptr.size());
+        UP(up_nown, ups.n_owned.data(), ups.n_owned.size());
+        UP(up_row_ptr, ups.row_ptr.data(), ups.row_ptr.size());
+        UP(up_rows, ups.rows.data(), ups.rows.size());
+        UP(mp_row_ptr, mps.row_ptr.data(), mps.row_ptr.size());
+        UP(mp_rows, mps.rows.data(), mps.rows.size());
+        UP(mp_items, mps.items.data(), mps.items.size());
+        UP(mp_faces, mps.faces.data(), mps.faces.size());
+        UP(mp_item_ptr, mps.item_ptr.data(), mps.item_ptr.size());
+        UP(mp_face_ptr, mps.face_ptr.data(), mps.face_ptr.size());
+    }
     UP(vel_d, t_vd.data(), t_vd.size());
     UP(vel_f, t_vf.data(), t_vf.size());
     if (c->nt > 0) {
@@ -2037,6 +2191,17 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         if ((rc = dalloc(c, &ft, size_t(nf) * c->nt * es))) return fail(rc);
         c->face_tab = ft;
     }
+    if (c->patches) {
+        char *cr = nullptr;
+        const size_t rs = storage == MESO_STORAGE_FP32 ? sizeof(CellRec<float>) : sizeof(CellRec<double>);
+        if ((rc = dalloc(c, &cr, size_t(nc) * rs))) return fail(rc);
+        c->crec = cr;
+        if (c->nt > 0) {
+            char *ct = nullptr;
+            if ((rc = dalloc(c, &ct, size_t(nc) * 2 * c->nt * es))) return fail(rc);
+            c->cell_tab = ct;
+        }
+    }
     c->xi_h.assign(d->xi, d->xi + 3 * size_t(nv));
     c->w_h.assign(d->w, d->w + nv);
     if (hipHostMalloc((void **)&c->flags_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
@@ -2049,14 +2214,14 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
 
 void meso_hip_destroy(meso_hip_ctx *c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto &r : c->recs) {
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
     }
-    for (void *p : c->allocs) hipFree(p);
-    if (c->flags_host) hipHostFree(c->flags_host);
-    if (c->stream) hipStreamDestroy(c->stream);
+    for (void *p : c->allocs) (void)hipFree(p);
+    if (c->flags_host) (void)hipHostFree(c->flags_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -2103,7 +2268,7 @@ static int init_common(meso_hip_ctx *c, const double *rho, const double *u3, con
         if (!t) continue;
         auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)t);
         if (it != c->allocs.end()) c->allocs.erase(it);
-        hipFree(t);
+        (void)hipFree(t);
         c->bytes -= int64_t(std::max<size_t>(t == tu ? 3 * nc * 8 : nc * 8, 16));
     }
     c->cur = 1 - c->cur;
@@ -2134,7 +2299,7 @@ static int transpose(meso_hip_ctx *c, bool to_device, double *host, int which, i
     auto release = [&]() {
         auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)stage);
         if (it != c->allocs.end()) c->allocs.erase(it);
-        hipFree(stage);
+        (void)hipFree(stage);
         c->bytes -= int64_t(std::max<size_t>(n * 8, 16));
     };
     if (layout == MESO_LAYOUT_CELL_MAJOR) {  // host [c][v] -> stage [v][c] via CPU transpose
@@ -2300,8 +2465,8 @@ int meso_hip_profile(meso_hip_ctx *c, int32_t enable) {
     if (!c) return set_err(c, MESO_E_INVALID, "null context");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (auto &r : c->recs) {
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
     }
     c->recs.clear();
     c->totals.clear();
@@ -2315,9 +2480,9 @@ int meso_hip_kernel_times(meso_hip_ctx *c, int32_t max_entries, const char **nam
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     for (auto &r : c->recs) {
         float ms = 0.f;
-        hipEventElapsedTime(&ms, r.a, r.b);
-        hipEventDestroy(r.a);
-        hipEventDestroy(r.b);
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
         bool found = false;
         for (auto &t : c->totals)
             if (std::strcmp(t.first, r.name) == 0) {

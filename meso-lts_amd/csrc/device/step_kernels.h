@@ -23,6 +23,7 @@
 // f^{n+1} once.
 #pragma once
 
+#include "patches.h"
 #include "step_common.h"
 
 namespace meso_dev {
@@ -108,6 +109,21 @@ struct KArgs {
     int *bad_step;
     int *halt;
     int step_index;
+    // LDS-staged patch kernels (2D): update patches (interior cells) and
+    // moment patches (all faces); velocity range split into p_chunks chunks of
+    // p_tpc tiles
+    int n_up, up_maxi, up_maxf, up_maxo, up_maxr;
+    const meso_patch::Item *up_items;
+    const meso_patch::Face *up_faces;
+    const int *up_item_ptr, *up_face_ptr, *up_nown, *up_row_ptr, *up_rows;
+    int n_mp, mp_maxi, mp_maxf, mp_maxr;
+    const meso_patch::Item *mp_items;
+    const meso_patch::Face *mp_faces;
+    const int *mp_item_ptr, *mp_face_ptr, *mp_row_ptr, *mp_rows;
+    int p_chunks, p_tpc;
+    int n_int_cells;            // interior cells: cell_list[0, n_int_cells)
+    void *crec;                 // [nc] CellRec of the storage type
+    void *cell_tab;             // [nc][2][nt] axis tables of the cell equilibria
     // FP32 storage keeps f - f_ref; f_ref is the equilibrium `ref` at rest.
     EqParams ref;
     const double *fref_d;  // [2][nv] g_ref then h_ref (FP64)
