@@ -40,8 +40,8 @@ struct CellRec<double> {
 };
 
 // ------------------------------------------------------------------- prep
-// Per interior cell: conservative macro update (as k_cell_update does for its
-// cells), the constants of the f update, and the cell's axis tables.
+// Per cell of the update patches: conservative macro update (as k_cell_update
+// does for its cells), the constants of the f update, the cell's axis tables.
 template <int MODEL, class S, bool TAB>
 __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
     if (*a.halt) return;
@@ -216,6 +216,38 @@ __device__ __forceinline__ void face_eq_lane(const KArgs &a, int f, const LaneVe
     }
 }
 
+// Value of boundary face f at the lane velocity, in the storage form, from the
+// mixed value fm (gm, hm) there: the incoming half takes the boundary
+// equilibrium (FP64), the outgoing half keeps fm; on the dead-band a wall takes
+// the average (cdugks_shakhov.cpp:364-607). Mirror boundaries (symmetry,
+// slip-wall) never reach this kernel.
+template <int MODEL, class S>
+__device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn, const LaneVel &L, double gm,
+                                               double hm, double &g, double &h) {
+    g = gm;
+    h = hm;
+    const int bi = a.face_bnd[f];
+    if (bi < 0) return;
+    const BndParams &P = a.bnd[bi];
+    const double kn = -sn;  // xi . n1, n1 = -n0 points into the domain
+    if (P.kind == BC_INTERIOR || kn < -VSMALL) return;
+    const double x[3] = {L.x, L.y, 0.0};
+    double gi, hi = 0.0;
+    if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<double>(P.in, x, 2, a.s.K, gi, hi);
+    else gi = bgk_eq<double>(P.in, x);
+    if constexpr (sizeof(S) == 4) {
+        gi -= L.gref_d;
+        hi -= L.href_d;
+    }
+    if (kn >= VSMALL) {
+        g = gi;
+        h = hi;
+    } else if (P.kind == BC_WALL) {
+        g = 0.5 * (gi + gm);
+        h = 0.5 * (hi + hm);
+    }
+}
+
 // ------------------------------------------------------------- moments
 // Raw moments of f-bar+ per face of the moment patches, summed per velocity
 // chunk into part_f[face][chunk][col] (the layout k_face_eq reads).
@@ -356,8 +388,8 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_moments(KArgs a) {
 }
 
 // ------------------------------------------------------------- update
-// Interior cells of the update patches: cell flux, f-tilde-plus update and the
-// Shakhov heat-flux partials part_c[cell][chunk][0..1].
+// Cells of the update patches: cell flux, f-tilde-plus update and the Shakhov
+// heat-flux partials part_c[cell][chunk][0..1].
 template <int MODEL, class S, int VT, bool TAB>
 __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -419,27 +451,32 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 const int lf = e & 0xffff, mode = e >> 16;
                 const meso_patch::Face *F = faces + lf;
                 const double sn = F->n[0] * L.x + F->n[1] * L.y;
+                const bool bnd = mode == meso_patch::SIDE_BOUNDARY;
                 const bool own = mode == meso_patch::SIDE_OWNER;
-                const bool up = own ? sn >= VSMALL : sn < -VSMALL;
                 const bool dead = !(sn >= VSMALL) && !(sn < -VSMALL);
+                const bool up = bnd || (own ? sn >= VSMALL : sn < -VSMALL);
                 if (!__any(up || (own && dead))) continue;
                 if (up) {
                     const int f = F->face;
                     double gs, hs = 0.0;
                     face_eq_lane<MODEL, S, TAB>(a, f, L, gs, hs);
                     const double mix = a.face_eq[f].mix;
-                    const int side = own ? 0 : 1;
+                    const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
                     const double ex = F->d[side][0] - L.x * hdt, ey = F->d[side][1] - L.y * hdt;
                     const double knA = sn * F->area;
+                    double fm[2] = {0.0, 0.0};
 #pragma unroll
                     for (int d = 0; d < ND; ++d) {
-                        const double fbar = fc[d] + (ex * gx[d] + ey * gy[d]);
+                        const double Xd = ex * gx[d] + ey * gy[d];
+                        // a boundary face's dead-band value 0.5 (f + f + X + X) is f + X
+                        const double fbar = fc[d] + Xd;
                         const double eq = d == 0 ? gs : hs;
-                        double fm;
-                        if constexpr (sizeof(S) == 8) fm = (1.0 - mix) * fbar + mix * eq;
-                        else fm = fbar + mix * (eq - fbar);
-                        flx[(d * maxf + lf) * VT + vl] = S(knA * fm);
+                        if constexpr (sizeof(S) == 8) fm[d] = (1.0 - mix) * fbar + mix * eq;
+                        else fm[d] = fbar + mix * (eq - fbar);
                     }
+                    if (bnd) bnd_value_lane<MODEL, S>(a, F->face, sn, L, fm[0], fm[1], fm[0], fm[1]);
+#pragma unroll
+                    for (int d = 0; d < ND; ++d) flx[(d * maxf + lf) * VT + vl] = S(knA * fm[d]);
                 } else if (own && dead) {
 #pragma unroll
                     for (int d = 0; d < ND; ++d) flx[(d * maxf + lf) * VT + vl] = S(0);
@@ -458,11 +495,11 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 if (k >= nface) continue;
                 const int e = it->face[k];
                 const int lf = e & 0xffff;
-                const bool own = (e >> 16) == meso_patch::SIDE_OWNER;
+                const bool nei = (e >> 16) == meso_patch::SIDE_NEIGHBOUR;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                     const double fx = double(flx[(d * maxf + lf) * VT + vl]);
-                    F[d] += own ? fx : -fx;
+                    F[d] += nei ? -fx : fx;
                 }
             }
             const CellRec<S> &R = reinterpret_cast<const CellRec<S> *>(a.crec)[c];

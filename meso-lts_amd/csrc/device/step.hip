@@ -1289,6 +1289,9 @@ struct meso_hip_ctx {
     int *cell_list = nullptr, *st_id = nullptr, *fs_id = nullptr;
     double *st_a = nullptr, *fs_a = nullptr;
     float *st_af = nullptr, *fs_af = nullptr;
+    // cell_list = [first group | second group]: without patch kernels the interior
+    // cells, then the cells with a boundary face; with patch kernels the cells
+    // of the update patches, then the cells with a mirror boundary
     int n_int = 0, n_bcell = 0;
     // patch kernels (2D, large velocity sets)
     bool patches = false;
@@ -2051,8 +2054,17 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         std::string perr;
         bool ok = true;
         const size_t si = sizeof(meso_patch::Item), sf = sizeof(meso_patch::Face);
+        // every cell is updated by a patch except those with a mirror boundary
+        // (symmetry, slip-wall), whose face values need other velocities
+        std::vector<char> pown(nc, 1);
+        for (int f = 0; f < nf; ++f) {
+            const int b = c->face_bnd_h[f];
+            if (b < 0) continue;
+            const int kind = m->marks[m->face_mark[f]].type;
+            if (kind == MESO_BC_SYMMETRY || kind == MESO_BC_SLIP_WALL) pown[m->face_cells[2 * f]] = 0;
+        }
         for (int P = env_int("MESO_HIP_PATCH_CELLS", 32); ok; P = P * 3 / 4) {
-            ok = meso_patch::build_update_patches(mv, P, ups, perr);
+            ok = meso_patch::build_update_patches(mv, pown, P, ups, perr);
             // items | faces | rows | values | face fluxes | q partials (k_patch_update)
             c->lds_up = r16(ups.max_items * si + ups.max_faces * sf + 4 * size_t(ups.max_rows)) +
                         r16(size_t(ND) * ups.max_rows * VT * es + size_t(ND) * ups.max_faces * VT * es) +
@@ -2087,6 +2099,14 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             c->mp_maxi = mps.max_items;
             c->mp_maxf = mps.max_faces;
             c->mp_maxr = mps.max_rows;
+            // cell lists: patch cells first (k_cell_prep), then the mirror-boundary cells (k_cell_update)
+            t_list.clear();
+            for (int cc = 0; cc < nc; ++cc)
+                if (pown[cc]) t_list.push_back(cc);
+            c->n_int = int(t_list.size());
+            for (int cc = 0; cc < nc; ++cc)
+                if (!pown[cc]) t_list.push_back(cc);
+            c->n_bcell = nc - c->n_int;
         }
     }
     {

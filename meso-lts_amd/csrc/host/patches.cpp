@@ -159,20 +159,15 @@ std::vector<std::vector<int>> rcb(const std::vector<int> &ids, const double *pos
     return out;
 }
 
-bool build_update_patches(const MeshView &m, int max_owned, PatchSet &ps, std::string &err) {
+bool build_update_patches(const MeshView &m, const std::vector<char> &own, int max_owned, PatchSet &ps,
+                          std::string &err) {
     reset(ps);
     if (!check_limits(m, err)) return false;
-    std::vector<int> interior;
-    for (int c = 0; c < m.nc; ++c) {
-        bool bnd = false;
-        for (int k = m.cf_ptr[c]; k < m.cf_ptr[c + 1]; ++k) {
-            const int f = m.cf_ids[k];
-            bnd = bnd || m.face_cells[2 * size_t(f)] == m.face_cells[2 * size_t(f) + 1];
-        }
-        if (!bnd) interior.push_back(c);
-    }
+    std::vector<int> owned;
+    for (int c = 0; c < m.nc; ++c)
+        if (own[c]) owned.push_back(c);
     Builder b(m, ps);
-    for (const auto &grp : rcb(interior, m.cell_pos, 3, max_owned)) {
+    for (const auto &grp : rcb(owned, m.cell_pos, 3, max_owned)) {
         for (int c : grp) b.add_item(c);
         // faces of the owned cells in meso's cell-face order, and the cells across them
         for (int c : grp)

@@ -5,9 +5,10 @@
 // values of a patch's "items" (the cells whose value and least-squares
 // gradient it needs) in LDS once per velocity tile, forms every gradient
 // and every face value once, and reuses them from LDS. Two patch sets exist:
-//   update patches: owned cells = interior cells (no boundary face) that the
-//     patch updates; items = owned cells, then the cells across their faces;
-//     faces = all faces of the owned cells.
+//   update patches: owned cells = the cells the patch updates (all but those
+//     whose boundary needs other velocities, i.e. mirror boundaries); items =
+//     owned cells, then the cells across their faces; faces = all faces of
+//     the owned cells.
 //   moment patches: faces = a compact group of faces (boundary ones included)
 //     whose f-bar+ moments the patch sums; items = owners and neighbours of
 //     those faces.
@@ -78,10 +79,12 @@ struct MeshView {
     const double *nb_a = nullptr;                    // [nnz * 3] gradient weights
 };
 
-// Interior cells (no boundary face) grouped into update patches of at most
-// `max_owned` cells. Returns false with a message if the mesh does not fit
-// the 2D limits (more than MAXF faces or MAXN neighbours per cell).
-bool build_update_patches(const MeshView &m, int max_owned, PatchSet &out, std::string &err);
+// The cells flagged in `own` grouped into update patches of at most
+// `max_owned` cells (their boundary faces become SIDE_BOUNDARY entries).
+// Returns false with a message if the mesh does not fit the 2D limits (more
+// than MAXF faces or MAXN neighbours per cell).
+bool build_update_patches(const MeshView &m, const std::vector<char> &own, int max_owned, PatchSet &out,
+                          std::string &err);
 
 // All faces grouped into moment patches of at most `max_faces` faces.
 bool build_moment_patches(const MeshView &m, int max_faces, PatchSet &out, std::string &err);
