@@ -112,17 +112,29 @@ __device__ __forceinline__ void item_grad(const meso_patch::Item *it, const S *v
     }
 }
 
-// Block prologue: copy the patch's items, faces and stage-row cell ids to LDS.
-__device__ __forceinline__ void stage_records(const meso_patch::Item *gi, int ni, const meso_patch::Face *gf, int nfp,
-                                              const int *gr, int nr, meso_patch::Item *li, meso_patch::Face *lf,
-                                              int *lr) {
-    const int4 *si = reinterpret_cast<const int4 *>(gi);
-    int4 *di = reinterpret_cast<int4 *>(li);
-    for (int r = threadIdx.x; r < ni * int(sizeof(meso_patch::Item) / 16); r += BLOCKP) di[r] = si[r];
-    const int4 *sf = reinterpret_cast<const int4 *>(gf);
-    int4 *df = reinterpret_cast<int4 *>(lf);
-    for (int r = threadIdx.x; r < nfp * int(sizeof(meso_patch::Face) / 16); r += BLOCKP) df[r] = sf[r];
-    for (int r = threadIdx.x; r < nr; r += BLOCKP) lr[r] = gr[r];
+// Block prologue: copy n records of R bytes (a multiple of 16) to LDS.
+template <class R>
+__device__ __forceinline__ void stage_copy(const R *src, int n, R *dst) {
+    const int4 *si = reinterpret_cast<const int4 *>(src);
+    int4 *di = reinterpret_cast<int4 *>(dst);
+    for (int r = threadIdx.x; r < n * int(sizeof(R) / 16); r += BLOCKP) di[r] = si[r];
+}
+
+// Patch records: with one item per wave (IPW == 1) every record a wave reads
+// is wave-uniform and is read from HBM through the scalar cache; with several
+// items per wave the records are staged in LDS.
+template <int IPW>
+struct Records {
+    const meso_patch::Item *items;
+    const meso_patch::Face *faces;
+    __device__ __forceinline__ const meso_patch::Item *item(int s) const { return items + s; }
+    __device__ __forceinline__ const meso_patch::Face *face(int lf) const { return faces + lf; }
+};
+
+// Bytes of LDS the records take (0 when they are read from HBM).
+template <int IPW>
+__host__ __device__ constexpr size_t record_bytes(int maxi, int maxf) {
+    return IPW > 1 ? size_t(maxi) * sizeof(meso_patch::Item) + size_t(maxf) * sizeof(meso_patch::Face) : 0;
 }
 
 // Stage the rows of one velocity tile: all loads of a batch are issued before
@@ -252,7 +264,7 @@ __device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn,
 // Raw moments of f-bar+ per face of the moment patches, summed per velocity
 // chunk into part_f[face][chunk][col] (the layout k_face_eq reads).
 template <int MODEL, class S, int VT>
-__global__ __launch_bounds__(BLOCKP) void k_patch_moments(KArgs a) {
+__global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.bad_count > 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) *a.halt = 1;
@@ -272,30 +284,43 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_moments(KArgs a) {
     const int f0 = a.mp_face_ptr[p], nfp = a.mp_face_ptr[p + 1] - f0;
     const int r0 = a.mp_row_ptr[p], nr = a.mp_row_ptr[p + 1] - r0;
     const int maxi = a.mp_maxi, maxf = a.mp_maxf, maxr = a.mp_maxr;
-    // LDS: items | faces | rows | tile velocities | values | f-bar+ rows
-    auto *items = reinterpret_cast<meso_patch::Item *>(lds_raw);
-    auto *faces = reinterpret_cast<meso_patch::Face *>(items + maxi);
-    int *rows = reinterpret_cast<int *>(faces + maxf);
-    double4 *vtile = reinterpret_cast<double4 *>(lds_raw + ((reinterpret_cast<unsigned char *>(rows + maxr) - lds_raw + 31) & ~31));
+    // LDS: [items | faces] | rows | tile velocities | values | f-bar+ rows
+    unsigned char *q = lds_raw;
+    Records<IPW> rec;
+    if constexpr (IPW > 1) {
+        auto *li = reinterpret_cast<meso_patch::Item *>(q);
+        auto *lf = reinterpret_cast<meso_patch::Face *>(li + maxi);
+        stage_copy(a.mp_items + i0, ni, li);
+        stage_copy(a.mp_faces + f0, nfp, lf);
+        rec.items = li;
+        rec.faces = lf;
+        q += record_bytes<IPW>(maxi, maxf);
+    } else {
+        rec.items = a.mp_items + i0;
+        rec.faces = a.mp_faces + f0;
+    }
+    int *rows = reinterpret_cast<int *>(q);
+    q += (size_t(maxr) * 4 + 31) & ~size_t(31);
+    double4 *vtile = reinterpret_cast<double4 *>(q);
     S *val = reinterpret_cast<S *>(vtile + VT);
     S *fb = val + size_t(ND) * maxr * VT;
-    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    const int sub = lane / VT, vl = lane % VT;
+    for (int r = threadIdx.x; r < nr; r += BLOCKP) rows[r] = a.mp_rows[r0 + r];
+    const int lane = threadIdx.x % 64;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x / 64));
+    const int sub = IPW > 1 ? lane / VT : 0, vl = IPW > 1 ? lane % VT : lane;
     const int slot = wave * IPW + sub;
     const int nvp = a.s.nvp;
     const double hdt = a.s.half_dt;
     const S *fin_g = (const S *)a.fin_g, *fin_h = (const S *)a.fin_h;
-    stage_records(a.mp_items + i0, ni, a.mp_faces + f0, nfp, a.mp_rows + r0, nr, items, faces, rows);
-    // phase-C layout: lane = face of group fg; the waves of a group split the tile's velocities
-    const int nfg = (nfp + 63) / 64;
-    const int fg = wave % nfg, nvs = NW / nfg, vs = wave / nfg;
-    const int my_face = fg * 64 + lane;
-    const bool face_act = vs < nvs && my_face < nfp;
+    // phase-C layout: thread = (face, velocity subset); FG faces per subset
+    const int FG = maxf > 32 ? 64 : 32;
+    const int my_face = threadIdx.x % FG, vs = threadIdx.x / FG, nvs = BLOCKP / FG;
+    const bool face_act = my_face < nfp;
     double acc[NCOL];
 #pragma unroll
     for (int k = 0; k < NCOL; ++k) acc[k] = 0.0;
     const int ntiles = (a.s.nv + VT - 1) / VT;
-    const int t0 = chunk * a.p_tpc, t1 = min(t0 + a.p_tpc, ntiles);
+    const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
     __syncthreads();
     for (int t = t0; t < t1; ++t) {
         const int v = t * VT + vl;
@@ -311,7 +336,7 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_moments(KArgs a) {
         __syncthreads();
         // B: gradients and f-bar+ of the patch faces
         for (int s = slot; s < ni; s += NSLOT) {
-            const meso_patch::Item *it = items + s;
+            const meso_patch::Item *it = rec.item(s);
             const int nface = it->nface;
             double fc[2], gx[2], gy[2];
 #pragma unroll
@@ -322,16 +347,17 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_moments(KArgs a) {
             }
 #pragma unroll
             for (int k = 0; k < meso_patch::MAXF; ++k) {
-                if (k >= nface || !ok) continue;
+                if (k >= nface) continue;
                 const int e = it->face[k];
                 const int lf = e & 0xffff, mode = e >> 16;
-                const meso_patch::Face *F = faces + lf;
+                const meso_patch::Face *F = rec.face(lf);
                 const double sn = F->n[0] * X.x + F->n[1] * X.y;  // + n_z xi_z, both 0 in 2D
                 const bool up = mode == meso_patch::SIDE_BOUNDARY ||
                                 (mode == meso_patch::SIDE_OWNER ? sn >= VSMALL : sn < -VSMALL);
                 const bool dead = !(sn >= VSMALL) && !(sn < -VSMALL);
                 const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
                 const double ex = F->d[side][0] - X.x * hdt, ey = F->d[side][1] - X.y * hdt;
+                if (!ok) continue;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                     const double Xd = ex * gx[d] + ey * gy[d];
@@ -347,7 +373,7 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_moments(KArgs a) {
             }
         }
         __syncthreads();
-        // C: moments, lane = face, this wave's share of the tile's velocities
+        // C: moments, thread = (face, velocity subset)
         if (face_act) {
             for (int vv = vs; vv < VT; vv += nvs) {
                 const double4 Y = vtile[vv];
@@ -373,15 +399,15 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_moments(KArgs a) {
         __syncthreads();
     }
     // combine the velocity subsets of each face in a fixed order
-    double *red = reinterpret_cast<double *>(lds_raw);  // [NW][64][NCOL]
+    double *red = reinterpret_cast<double *>(lds_raw);  // [nvs][FG][NCOL]
     if (face_act)
 #pragma unroll
-        for (int k = 0; k < NCOL; ++k) red[(size_t(vs) * nfg * 64 + my_face) * NCOL + k] = acc[k];
+        for (int k = 0; k < NCOL; ++k) red[(size_t(vs) * FG + my_face) * NCOL + k] = acc[k];
     __syncthreads();
     for (int r = threadIdx.x; r < nfp * NCOL; r += BLOCKP) {
         const int lf = r / NCOL, k = r - lf * NCOL;
         double sum = 0.0;
-        for (int q = 0; q < nvs; ++q) sum += red[(size_t(q) * nfg * 64 + lf) * NCOL + k];
+        for (int qq = 0; qq < nvs; ++qq) sum += red[(size_t(qq) * FG + lf) * NCOL + k];
         const int f = a.mp_faces[f0 + lf].face;
         a.part_f[(size_t(f) * a.s.ntile_f + chunk) * NCOL + k] = sum;
     }
@@ -422,10 +448,12 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     const double hdt = a.s.half_dt;
     const S *fin_g = (const S *)a.fin_g, *fin_h = (const S *)a.fin_h;
     S *fout_g = (S *)a.fout_g, *fout_h = (S *)a.fout_h;
-    stage_records(a.up_items + i0, ni, a.up_faces + f0, nfp, a.up_rows + r0, nr, items, faces, rows);
+    stage_copy(a.up_items + i0, ni, items);
+    stage_copy(a.up_faces + f0, nfp, faces);
+    for (int r = threadIdx.x; r < nr; r += BLOCKP) rows[r] = a.up_rows[r0 + r];
     for (int r = threadIdx.x; r < 2 * nown; r += BLOCKP) qacc[r] = 0.0;
     const int ntiles = (a.s.nv + VT - 1) / VT;
-    const int t0 = chunk * a.p_tpc, t1 = min(t0 + a.p_tpc, ntiles);
+    const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
     __syncthreads();
     for (int t = t0; t < t1; ++t) {
         const int v = t * VT + vl;

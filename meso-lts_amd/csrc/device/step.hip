@@ -147,6 +147,111 @@ __device__ __forceinline__ void bnd_value(const KArgs &a, const BndParams &P, in
     }
 }
 
+// FP64 boundary-face values from any storage, as full values (the stored
+// deviation of FP32 storage plus f_ref). k_boundary uses them for the wall
+// density sums and the exact boundary flux moments, so the conservative update
+// of boundary cells carries no FP32 rounding of weights or geometry.
+template <class T>
+__device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restrict__ fd, int k, int f, int c,
+                                               double sn, int v, const double xh[3], double eq_full, double mix) {
+    const int nvp = a.s.nvp;
+    const double fs = double(fd[size_t(c) * nvp + v]);
+    double fb;
+    const bool dead = !(sn >= VSMALL || sn < -VSMALL);
+    if (!a.s.grad) {
+        fb = dead ? 0.5 * (fs + fs) : fs;
+    } else {
+        double gr[3] = {0.0, 0.0, 0.0};
+        for (int j = a.nb_ptr[c]; j < a.nb_ptr[c + 1]; ++j) {
+            const double df = double(fd[size_t(a.nb_ids[j]) * nvp + v]) - fs;
+            gr[0] += a.nb_a[3 * j] * df;
+            gr[1] += a.nb_a[3 * j + 1] * df;
+            gr[2] += a.nb_a[3 * j + 2] * df;
+        }
+        const double X = (a.face_d[6 * f] - xh[0]) * gr[0] + (a.face_d[6 * f + 1] - xh[1]) * gr[1] +
+                         (a.face_d[6 * f + 2] - xh[2]) * gr[2];
+        fb = dead ? 0.5 * (fs + fs + X + X) : fs + X;
+    }
+    if constexpr (sizeof(T) == 4) fb += a.fref_d[k * a.s.nv + v];
+    return (1.0 - mix) * fb + mix * eq_full;
+}
+
+template <int MODEL, int D, class T>
+__device__ __forceinline__ void bnd_mix_pair_full(const KArgs &a, int f, int c, int v, const EqParams &ef, double &g,
+                                                  double &h) {
+    const double x[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
+    const double hd = a.s.half_dt;
+    const double xh[3] = {x[0] * hd, x[1] * hd, x[2] * hd};
+    const double sn = ndot(a, f, v);
+    double gs, hs = 0.0;
+    if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<double>(ef, x, D, a.s.K, gs, hs);
+    else gs = bgk_eq<double>(ef, x);
+    g = bnd_mix_full<T>(a, (const T *)a.fin_g, 0, f, c, sn, v, xh, gs, ef.mix);
+    h = MODEL == CDUGKS_SHAKHOV ? bnd_mix_full<T>(a, (const T *)a.fin_h, 1, f, c, sn, v, xh, hs, ef.mix) : 0.0;
+}
+
+// Final full value at velocity v of boundary face f from the mixed value there.
+template <int MODEL, int D, class T>
+__device__ __forceinline__ void bnd_value_full(const KArgs &a, const BndParams &P, int f, int c, int v, double sn,
+                                               const EqParams &ef, double gmix, double hmix, double &g, double &h) {
+    g = gmix;
+    h = hmix;
+    const double kn = -sn;
+    if (P.kind == BC_INTERIOR || kn < -VSMALL) return;
+    const double x[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
+    double gi = 0.0, hi = 0.0;
+    if (P.kind != BC_SYMMETRY) {
+        if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<double>(P.in, x, D, a.s.K, gi, hi);
+        else gi = bgk_eq<double>(P.in, x);
+    }
+    switch (P.kind) {
+        case BC_FARFIELD_INLET:
+        case BC_PRESSURE_INLET:
+        case BC_PRESSURE_OUTLET:
+        case BC_FARFIELD_OUTLET:
+            if (kn >= VSMALL) {
+                g = gi;
+                h = hi;
+            }
+            return;
+        case BC_WALL:
+            if (kn >= VSMALL) {
+                g = gi;
+                h = hi;
+            } else {
+                g = 0.5 * (gi + gmix);
+                h = 0.5 * (hi + hmix);
+            }
+            return;
+        case BC_SLIP_WALL:
+        case BC_SYMMETRY: {
+            const int vm = a.mirror[3 * v + P.dir];
+            double gr, hr;
+            bnd_mix_pair_full<MODEL, D, T>(a, f, c, vm, ef, gr, hr);
+            double gv, hv;
+            if (P.kind == BC_SLIP_WALL) {
+                gv = (1.0 - P.alpha_u) * gi + P.alpha_u * gr;
+                hv = (1.0 - P.alpha_T) * hi + P.alpha_T * hr;
+            } else {
+                gv = gr;
+                hv = hr;
+            }
+            if (kn >= VSMALL) {
+                g = gv;
+                h = hv;
+            } else if (P.kind == BC_SLIP_WALL) {
+                g = 0.5 * (gv + gmix);
+                h = 0.5 * (hv + hmix);
+            } else {
+                g = 0.5 * (gmix + gv);
+                h = 0.5 * (hmix + hv);
+            }
+            return;
+        }
+        default: return;
+    }
+}
+
 // ================================================================ kernels
 // Sign of n0 . xi against the +-VSMALL dead-band (meso's upwind test,
 // cdugks_shakhov.cpp:227-254). FP64 storage evaluates it in FP64 exactly as
@@ -588,9 +693,9 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
                     else gm = bgk_eq<double>(e1, X);
                     s[1] += kn * a.w[v] * gm;
                 } else {
-                    T gmx, hmx;
-                    bnd_mix_pair<MODEL, D, T>(a, f, c, v, ef, gmx, hmx);
-                    s[0] -= kn * a.w[v] * full_value<T>(a, gmx, 0, v);
+                    double gmx, hmx;
+                    bnd_mix_pair_full<MODEL, D, T>(a, f, c, v, ef, gmx, hmx);
+                    s[0] -= kn * a.w[v] * gmx;
                 }
             }
             block_sum<2>(s, lds);
@@ -610,19 +715,18 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
 #pragma unroll
     for (int k = 0; k < NFL; ++k) acc[k] = 0.0;
     for (int v = threadIdx.x; v < nv; v += BLOCK) {
-        T gmx, hmx, g, h;
-        bnd_mix_pair<MODEL, D, T>(a, f, c, v, ef, gmx, hmx);
+        double gmx, hmx, g, h;
+        bnd_mix_pair_full<MODEL, D, T>(a, f, c, v, ef, gmx, hmx);
         const double sn = ndot(a, f, v);
-        bnd_value<MODEL, D, T>(a, P, f, c, v, sn, ef, gmx, hmx, g, h);
+        bnd_value_full<MODEL, D, T>(a, P, f, c, v, sn, ef, gmx, hmx, g, h);
         const double X0 = a.xi[3 * v], X1 = a.xi[3 * v + 1], X2 = a.xi[3 * v + 2];
         const double wsn = a.w[v] * sn;
-        const double wg = wsn * full_value<T>(a, g, 0, v);
+        const double wg = wsn * g;
         acc[0] += wg;
         acc[1] += wg * X0;
         acc[2] += wg * X1;
         if (D == 3) acc[3] += wg * X2;
-        if (MODEL == CDUGKS_SHAKHOV)
-            acc[D + 1] += wg * (X0 * X0 + X1 * X1 + X2 * X2) + wsn * full_value<T>(a, h, 1, v);
+        if (MODEL == CDUGKS_SHAKHOV) acc[D + 1] += wg * (X0 * X0 + X1 * X1 + X2 * X2) + wsn * h;
     }
     block_sum<NFL>(acc, lds);
     if (threadIdx.x == 0) {
@@ -1297,7 +1401,7 @@ struct meso_hip_ctx {
     bool patches = false;
     int n_up = 0, up_maxi = 0, up_maxf = 0, up_maxo = 0, up_maxr = 0, n_mp = 0, mp_maxi = 0, mp_maxf = 0,
         mp_maxr = 0;
-    int p_chunks = 0, p_tpc = 0, p_vt = 0;
+    int p_chunks = 0, p_vchunk = 0;
     size_t lds_up = 0, lds_mp = 0;
     meso_patch::Item *up_items = nullptr, *mp_items = nullptr;
     meso_patch::Face *up_faces = nullptr, *mp_faces = nullptr;
@@ -1496,7 +1600,7 @@ KArgs make_args(meso_hip_ctx *c) {
     a.mp_item_ptr = c->mp_item_ptr;
     a.mp_face_ptr = c->mp_face_ptr;
     a.p_chunks = c->p_chunks;
-    a.p_tpc = c->p_tpc;
+    a.p_vchunk = c->p_vchunk;
     a.n_int_cells = c->n_int;
     a.crec = c->crec;
     a.cell_tab = c->cell_tab;
@@ -1626,15 +1730,16 @@ template <int MODEL, class T, bool TAB>
 int step_patch(meso_hip_ctx *c, KArgs a) {
     const StepConst &s = c->s;
     constexpr int WPB = BLOCK / 64;
-    constexpr int VT = sizeof(T) == 4 ? 32 : 16;
+    constexpr int VTM = sizeof(T) == 4 ? 64 : 32;  // velocities per tile: moments
+    constexpr int VTU = sizeof(T) == 4 ? 32 : 16;  // velocities per tile: update
     constexpr int MAXF = 4;
     const int fwblk = (s.nf + WPB - 1) / WPB;
-    LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VT>), c->n_mp * c->p_chunks, c->lds_mp, a);
+    LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM>), c->n_mp * c->p_chunks, c->lds_mp, a);
     LAUNCH_SH(c, "face_eq", (k_face_eq<MODEL, 2, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
     if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, 2, T>), s.nb, 0, a);
     if (c->n_int > 0) {
         LAUNCH(c, "cell_prep", (k_cell_prep<MODEL, T, TAB>), (c->n_int + WPB - 1) / WPB, a);
-        LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VT, TAB>), c->n_up * c->p_chunks, c->lds_up, a);
+        LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB>), c->n_up * c->p_chunks, c->lds_up, a);
     }
     const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
     if (c->n_bcell > 0)
@@ -2031,7 +2136,12 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     // ---- LDS-staged patch kernels: 2D CDUGKS models with large velocity sets
     meso_patch::PatchSet ups, mps;
     if (D == 2 && p->model != MESO_MODEL_DUGKS_BGK && nv >= 64 && env_int("MESO_HIP_PATCH", 1) != 0) {
-        const int VT = storage == MESO_STORAGE_FP32 ? 32 : 16;
+        // velocities per tile (must match step_patch): the moment kernel uses
+        // 64 for FP32 storage (one item per wave, records read through the
+        // scalar cache), the update kernel 32 (two items per wave, records in
+        // LDS); FP64 storage halves both
+        const int VTM = storage == MESO_STORAGE_FP32 ? 64 : 32, VTU = VTM / 2;
+        const int IPWM = 64 / VTM, IPWU = 64 / VTU;
         const size_t es = storage == MESO_STORAGE_FP32 ? 4 : 8;
         const int ND = c->nd;
         const int ncol = c->nd == 2 ? 11 : 6;
@@ -2063,33 +2173,32 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             const int kind = m->marks[m->face_mark[f]].type;
             if (kind == MESO_BC_SYMMETRY || kind == MESO_BC_SLIP_WALL) pown[m->face_cells[2 * f]] = 0;
         }
+        auto rec = [&](int ipw, int mi, int mf) { return ipw > 1 ? size_t(mi) * si + size_t(mf) * sf : size_t(0); };
         for (int P = env_int("MESO_HIP_PATCH_CELLS", 32); ok; P = P * 3 / 4) {
             ok = meso_patch::build_update_patches(mv, pown, P, ups, perr);
-            // items | faces | rows | values | face fluxes | q partials (k_patch_update)
-            c->lds_up = r16(ups.max_items * si + ups.max_faces * sf + 4 * size_t(ups.max_rows)) +
-                        r16(size_t(ND) * ups.max_rows * VT * es + size_t(ND) * ups.max_faces * VT * es) +
+            // [items | faces] | rows | values | face fluxes | q partials (k_patch_update)
+            c->lds_up = rec(IPWU, ups.max_items, ups.max_faces) + r16(4 * size_t(ups.max_rows)) +
+                        r16(size_t(ND) * ups.max_rows * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
                         size_t(ups.max_owned) * 16;
             if (c->lds_up <= cap || P <= 4) break;
         }
-        for (int F = env_int("MESO_HIP_PATCH_FACES", 64); ok; F = F * 3 / 4) {
+        const int fcap = env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
+        for (int F = std::min(fcap, 64); ok; F = F * 3 / 4) {
             ok = meso_patch::build_moment_patches(mv, F, mps, perr);
-            const int nfg = (mps.max_faces + 63) / 64;
-            // items | faces | rows | tile velocities | values | f-bar+ rows (k_patch_moments)
-            const size_t a1 = ((mps.max_items * si + mps.max_faces * sf + 4 * size_t(mps.max_rows) + 31) & ~size_t(31)) +
-                              size_t(VT) * 32 + size_t(ND) * mps.max_rows * VT * es +
-                              size_t(ND) * mps.max_faces * (VT + 1) * es;
-            const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer, any face-group count
+            // [items | faces] | rows | tile velocities | values | f-bar+ rows (k_patch_moments)
+            const size_t a1 = rec(IPWM, mps.max_items, mps.max_faces) +
+                              ((4 * size_t(mps.max_rows) + 31) & ~size_t(31)) + size_t(VTM) * 32 +
+                              size_t(ND) * mps.max_rows * VTM * es + size_t(ND) * mps.max_faces * (VTM + 1) * es;
+            const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer [threads][ncol]
             c->lds_mp = std::max(a1, a2);
-            if ((c->lds_mp <= cap && nfg <= NWP) || F <= 4) break;
+            if (c->lds_mp <= cap || F <= 4) break;
         }
         if (ok && c->lds_up <= cap && c->lds_mp <= cap && ups.n_patch > 0 && mps.n_patch > 0) {
             c->patches = true;
-            c->p_vt = VT;
-            const int ntiles = (nv + VT - 1) / VT;
-            c->p_tpc = std::max(1, (env_int("MESO_HIP_PCHUNK", 2560) + VT - 1) / VT);
-            c->p_chunks = (ntiles + c->p_tpc - 1) / c->p_tpc;
+            c->p_vchunk = std::max(64, (env_int("MESO_HIP_PCHUNK", 2560) + 63) / 64 * 64);
+            c->p_chunks = (nv + c->p_vchunk - 1) / c->p_vchunk;
             c->s.ntile_c = c->s.ntile_f = c->p_chunks;
-            c->s.vt_c = c->s.vt_f = c->p_tpc * VT;
+            c->s.vt_c = c->s.vt_f = c->p_vchunk;
             c->n_up = ups.n_patch;
             c->up_maxi = ups.max_items;
             c->up_maxf = ups.max_faces;

@@ -110,8 +110,8 @@ struct KArgs {
     int *halt;
     int step_index;
     // LDS-staged patch kernels (2D): update patches (interior cells) and
-    // moment patches (all faces); velocity range split into p_chunks chunks of
-    // p_tpc tiles
+    // moment patches (all faces); the velocity range is split into p_chunks
+    // chunks of p_vchunk velocities
     int n_up, up_maxi, up_maxf, up_maxo, up_maxr;
     const meso_patch::Item *up_items;
     const meso_patch::Face *up_faces;
@@ -120,7 +120,7 @@ struct KArgs {
     const meso_patch::Item *mp_items;
     const meso_patch::Face *mp_faces;
     const int *mp_item_ptr, *mp_face_ptr, *mp_row_ptr, *mp_rows;
-    int p_chunks, p_tpc;
+    int p_chunks, p_vchunk;     // velocity chunks of the patch kernels and their size (a multiple of 64)
     int n_int_cells;            // interior cells: cell_list[0, n_int_cells)
     void *crec;                 // [nc] CellRec of the storage type
     void *cell_tab;             // [nc][2][nt] axis tables of the cell equilibria
