@@ -1291,7 +1291,7 @@ __global__ __launch_bounds__(BLOCK) void k_init_macros(KArgs a) {
 // FP64 velocity-major [nv][nc] <-> device cell-major [nc][nvp] (32x32 tiles through LDS)
 template <class T, bool TO_DEVICE>
 __global__ __launch_bounds__(BLOCK) void k_transpose(const double *vm_in, double *vm_out, const T *cm_in, T *cm_out,
-                                                     int nc, int nv, int nvp, const double *fref) {
+                                                     int nc, int nv, int nvp, const double *fref, const int *slot_of) {
     __shared__ double tile[32][33];
     const int c0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
     const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
@@ -1303,12 +1303,16 @@ __global__ __launch_bounds__(BLOCK) void k_transpose(const double *vm_in, double
         __syncthreads();
         for (int k = ty; k < 32; k += 8) {
             int c = c0 + k, v = v0 + tx;
-            if (c < nc && v < nv) cm_out[size_t(c) * nvp + v] = T(tile[tx][k] - (fref ? fref[v] : 0.0));
+            if (c < nc && v < nv) {
+                const int sl = slot_of ? slot_of[v] : v;
+                cm_out[size_t(c) * nvp + sl] = T(tile[tx][k] - (fref ? fref[sl] : 0.0));
+            }
         }
     } else {
         for (int k = ty; k < 32; k += 8) {
             int c = c0 + k, v = v0 + tx;
-            tile[tx][k] = (c < nc && v < nv) ? double(cm_in[size_t(c) * nvp + v]) + (fref ? fref[v] : 0.0) : 0.0;
+            const int sl = (v < nv && slot_of) ? slot_of[v] : v;
+            tile[tx][k] = (c < nc && v < nv) ? double(cm_in[size_t(c) * nvp + sl]) + (fref ? fref[sl] : 0.0) : 0.0;
         }
         __syncthreads();
         for (int k = ty; k < 32; k += 8) {
@@ -1409,6 +1413,7 @@ struct meso_hip_ctx {
         *mp_face_ptr = nullptr, *up_row_ptr = nullptr, *up_rows = nullptr, *mp_row_ptr = nullptr,
         *mp_rows = nullptr;
     void *crec = nullptr, *cell_tab = nullptr;
+    int *slot_of = nullptr;  // meso's velocity index -> storage slot (null: identity)
     int *cf_ptr = nullptr, *cf_ids = nullptr, *nb_ptr = nullptr, *nb_ids = nullptr, *face_bnd = nullptr,
         *bnd_face = nullptr, *bnd_mark = nullptr, *mirror = nullptr;
     meso_mark_desc *marks = nullptr;
@@ -2133,6 +2138,63 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             }
         }
     }
+    // ---- velocity order. A tile of consecutive velocity slots should be compact
+    // in velocity space: its axis-table entries are then few and shared with the
+    // next tile, and most faces see one upwind side across it. A tensor-product
+    // set is therefore stored in Morton order of its axis indices; the state
+    // transfers map slots back to meso's order.
+    std::vector<int> perm(nv), slot_of(nv);
+    for (int v = 0; v < nv; ++v) perm[v] = v;
+    if (c->nt > 0 && env_int("MESO_HIP_VORDER", 1) != 0) {
+        int axis0[3] = {0, 0, 0};
+        for (int t = 1; t < c->nt; ++t)
+            if (t_tax[t] != t_tax[t - 1]) axis0[t_tax[t]] = t;
+        auto spread = [D](uint64_t x) {  // bit b of x to bit D*b
+            uint64_t r = 0;
+            for (int b = 0; b < 21; ++b) r |= ((x >> b) & 1ull) << (D * b);
+            return r;
+        };
+        std::vector<uint64_t> key(nv);
+        for (int v = 0; v < nv; ++v) {
+            uint64_t k = 0;
+            for (int dd = 0; dd < D; ++dd) k |= spread(uint64_t(((t_vtab[v] >> (10 * dd)) & 1023) - axis0[dd])) << dd;
+            key[v] = k;
+        }
+        std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return key[x] < key[y]; });
+    }
+    bool identity = true;
+    for (int sl = 0; sl < nv; ++sl) {
+        slot_of[perm[sl]] = sl;
+        identity = identity && perm[sl] == sl;
+    }
+    std::vector<double> xi_s(3 * size_t(nv)), w_s(nv);
+    std::vector<int> mirror_s;
+    {
+        std::vector<double4> vd(nv);
+        std::vector<float4> vf(nv);
+        std::vector<float> xf(3 * size_t(nv));
+        std::vector<int> vt(t_vtab.empty() ? 0 : nv);
+        for (int sl = 0; sl < nv; ++sl) {
+            const int v = perm[sl];
+            vd[sl] = t_vd[v];
+            vf[sl] = t_vf[v];
+            for (int k = 0; k < 3; ++k) {
+                xi_s[3 * size_t(sl) + k] = d->xi[3 * size_t(v) + k];
+                xf[3 * size_t(sl) + k] = xif[3 * size_t(v) + k];
+            }
+            w_s[sl] = d->w[v];
+            if (!vt.empty()) vt[sl] = t_vtab[v];
+        }
+        t_vd.swap(vd);
+        t_vf.swap(vf);
+        xif.swap(xf);
+        t_vtab.swap(vt);
+        if (d->mirror) {
+            mirror_s.resize(3 * size_t(nv));
+            for (int sl = 0; sl < nv; ++sl)
+                for (int k = 0; k < 3; ++k) mirror_s[3 * size_t(sl) + k] = slot_of[d->mirror[3 * size_t(perm[sl]) + k]];
+        }
+    }
     // ---- LDS-staged patch kernels: 2D CDUGKS models with large velocity sets
     meso_patch::PatchSet ups, mps;
     if (D == 2 && p->model != MESO_MODEL_DUGKS_BGK && nv >= 64 && env_int("MESO_HIP_PATCH", 1) != 0) {
@@ -2285,10 +2347,11 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     UP(bnd_face, c->bnd_face_h.data(), c->bnd_face_h.size());
     UP(bnd_mark, bmark.data(), bmark.size());
     UP(marks, m->marks, m->n_marks);
-    UP(xi, d->xi, 3 * size_t(nv));
+    UP(xi, xi_s.data(), xi_s.size());
     UP(xif, xif.data(), xif.size());
-    UP(w, d->w, nv);
-    if (d->mirror) UP(mirror, d->mirror, 3 * size_t(nv));
+    UP(w, w_s.data(), w_s.size());
+    if (d->mirror) UP(mirror, mirror_s.data(), mirror_s.size());
+    if (!identity) UP(slot_of, slot_of.data(), slot_of.size());
 #undef UP
     const size_t es = storage == MESO_STORAGE_FP32 ? 4 : 8;
     const size_t fsz = size_t(nc) * c->s.nvp;
@@ -2331,8 +2394,8 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             c->cell_tab = ct;
         }
     }
-    c->xi_h.assign(d->xi, d->xi + 3 * size_t(nv));
-    c->w_h.assign(d->w, d->w + nv);
+    c->xi_h = xi_s;
+    c->w_h = w_s;
     if (hipHostMalloc((void **)&c->flags_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(set_err(nullptr, MESO_E_DEVICE, "hipHostMalloc failed"));
     if ((rc = reset_flags(c))) return fail(rc);
@@ -2446,19 +2509,19 @@ static int transpose(meso_hip_ctx *c, bool to_device, double *host, int which, i
         HIP_TRY(c, hipMemcpyAsync(stage, src, n * 8, hipMemcpyHostToDevice, c->stream));
         if (f32)
             hipLaunchKernelGGL((k_transpose<float, true>), grid, dim3(BLOCK), 0, c->stream, stage, nullptr, nullptr,
-                               (float *)dev, nc, nv, nvp, fref);
+                               (float *)dev, nc, nv, nvp, fref, c->slot_of);
         else
             hipLaunchKernelGGL((k_transpose<double, true>), grid, dim3(BLOCK), 0, c->stream, stage, nullptr, nullptr,
-                               (double *)dev, nc, nv, nvp, fref);
+                               (double *)dev, nc, nv, nvp, fref, c->slot_of);
         HIP_TRY(c, hipGetLastError());
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     } else {
         if (f32)
             hipLaunchKernelGGL((k_transpose<float, false>), grid, dim3(BLOCK), 0, c->stream, nullptr, stage,
-                               (const float *)dev, nullptr, nc, nv, nvp, fref);
+                               (const float *)dev, nullptr, nc, nv, nvp, fref, c->slot_of);
         else
             hipLaunchKernelGGL((k_transpose<double, false>), grid, dim3(BLOCK), 0, c->stream, nullptr, stage,
-                               (const double *)dev, nullptr, nc, nv, nvp, fref);
+                               (const double *)dev, nullptr, nc, nv, nvp, fref, c->slot_of);
         HIP_TRY(c, hipGetLastError());
         double *dst = host + size_t(which) * n;
         if (layout == MESO_LAYOUT_CELL_MAJOR) {
