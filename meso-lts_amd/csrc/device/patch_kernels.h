@@ -228,6 +228,58 @@ __device__ __forceinline__ void face_eq_lane(const KArgs &a, int f, const LaneVe
     }
 }
 
+// Face equilibrium record of the storage kind (FP32 copy or FP64).
+template <class S>
+struct UpdateRecs;
+template <>
+struct UpdateRecs<float> {
+    using Eq = EqF;
+    __device__ static EqF face_eq(const KArgs &a, int f) { return a.face_eqf[f]; }
+};
+template <>
+struct UpdateRecs<double> {
+    using Eq = EqParams;
+    __device__ static EqParams face_eq(const KArgs &a, int f) { return a.face_eq[f]; }
+};
+
+// Bytes of LDS the face and cell records of k_patch_update take with RL.
+template <class S>
+__host__ __device__ constexpr size_t update_rec_bytes(int maxf, int maxo) {
+    return size_t(maxf) * (sizeof(typename UpdateRecs<S>::Eq) + 8) + size_t(maxo) * (sizeof(CellRec<S>) + 8);
+}
+
+// face_eq_lane with the face's equilibrium record e at hand.
+template <int MODEL, class S, bool TAB>
+__device__ __forceinline__ void face_eq_rec(const KArgs &a, const typename UpdateRecs<S>::Eq &e, int f,
+                                            const LaneVel &L, double &gs, double &hs) {
+    if constexpr (sizeof(S) == 4) {
+        float g, h = 0.0f;
+        if (MODEL == CDUGKS_SHAKHOV) {
+            if (TAB)
+                shakhov_tab_f(e, reinterpret_cast<const float *>(a.face_tab) + size_t(f) * a.nt, L.vt, L.xf, 2, a.s.K,
+                              L.gref, g, h);
+            else
+                shakhov_devf(e, L.xf, 2, a.s.K, L.gref, g, h);
+        } else {
+            g = bgk_devf(e, L.xf);
+        }
+        gs = g;
+        hs = h;
+    } else {
+        const double x[3] = {L.x, L.y, 0.0};
+        if (MODEL == CDUGKS_SHAKHOV) {
+            if (TAB)
+                shakhov_tab_d(e, reinterpret_cast<const double *>(a.face_tab) + size_t(f) * a.nt, L.vt, x, 2, a.s.K,
+                              gs, hs);
+            else
+                shakhov_eq<double>(e, x, 2, a.s.K, gs, hs);
+        } else {
+            gs = bgk_eq<double>(e, x);
+            hs = 0.0;
+        }
+    }
+}
+
 // Value of boundary face f at the lane velocity, in the storage form, from the
 // mixed value fm (gm, hm) there: the incoming half takes the boundary
 // equilibrium (FP64), the outgoing half keeps fm; on the dead-band a wall takes
@@ -415,8 +467,11 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
 
 // ------------------------------------------------------------- update
 // Cells of the update patches: cell flux, f-tilde-plus update and the Shakhov
-// heat-flux partials part_c[cell][chunk][0..1].
-template <int MODEL, class S, int VT, bool TAB>
+// heat-flux partials part_c[cell][chunk][0..1]. RL: the equilibrium records of
+// the patch faces and the update constants of the owned cells are copied to
+// LDS once per block; otherwise every tile re-reads them per lane from HBM
+// (L2), which costs address arithmetic and L2 latency in every tile.
+template <int MODEL, class S, int VT, bool TAB, bool RL>
 __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.halt) return;
@@ -441,6 +496,13 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     S *flx = val + size_t(ND) * maxr * VT;
     double *qacc = reinterpret_cast<double *>(
         lds_raw + ((reinterpret_cast<unsigned char *>(flx + size_t(ND) * maxf * VT) - lds_raw + 15) & ~15));
+    // with RL: the face equilibria and mix weights of the patch faces and the
+    // update constants of the owned cells, copied once per block
+    using EqS = typename UpdateRecs<S>::Eq;
+    EqS *feq = reinterpret_cast<EqS *>(qacc + 2 * a.up_maxo);
+    double *fmix = reinterpret_cast<double *>(feq + maxf);
+    CellRec<S> *crl = reinterpret_cast<CellRec<S> *>(fmix + maxf);
+    float2 *csl = reinterpret_cast<float2 *>(crl + a.up_maxo);
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int sub = lane / VT, vl = lane % VT;
     const int slot = wave * IPW + sub;
@@ -452,6 +514,18 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     stage_copy(a.up_faces + f0, nfp, faces);
     for (int r = threadIdx.x; r < nr; r += BLOCKP) rows[r] = a.up_rows[r0 + r];
     for (int r = threadIdx.x; r < 2 * nown; r += BLOCKP) qacc[r] = 0.0;
+    if constexpr (RL) {
+        for (int lf = threadIdx.x; lf < nfp; lf += BLOCKP) {
+            const int f = a.up_faces[f0 + lf].face;
+            feq[lf] = UpdateRecs<S>::face_eq(a, f);
+            fmix[lf] = a.face_eq[f].mix;
+        }
+        for (int r = threadIdx.x; r < nown; r += BLOCKP) {
+            const int c = a.up_items[i0 + r].cell;
+            crl[r] = reinterpret_cast<const CellRec<S> *>(a.crec)[c];
+            csl[r] = make_float2(a.cell_S[3 * c], a.cell_S[3 * c + 1]);
+        }
+    }
     const int ntiles = (a.s.nv + VT - 1) / VT;
     const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
     __syncthreads();
@@ -487,8 +561,14 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 if (up) {
                     const int f = F->face;
                     double gs, hs = 0.0;
-                    face_eq_lane<MODEL, S, TAB>(a, f, L, gs, hs);
-                    const double mix = a.face_eq[f].mix;
+                    double mix;
+                    if constexpr (RL) {
+                        face_eq_rec<MODEL, S, TAB>(a, feq[lf], f, L, gs, hs);
+                        mix = fmix[lf];
+                    } else {
+                        face_eq_lane<MODEL, S, TAB>(a, f, L, gs, hs);
+                        mix = a.face_eq[f].mix;
+                    }
                     const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
                     const double ex = F->d[side][0] - L.x * hdt, ey = F->d[side][1] - L.y * hdt;
                     const double knA = sn * F->area;
@@ -530,7 +610,7 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                     F[d] += nei ? -fx : fx;
                 }
             }
-            const CellRec<S> &R = reinterpret_cast<const CellRec<S> *>(a.crec)[c];
+            const CellRec<S> &R = RL ? crl[s] : reinterpret_cast<const CellRec<S> *>(a.crec)[c];
             const double fcg = double(val[s * VT + vl]);
             const double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
             double og, oh = 0.0;
@@ -538,7 +618,8 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 double gsn, hsn, gs, hs;
                 if constexpr (sizeof(S) == 4) {
                     // f_ref flux through the cell: f_ref xi . sum(+-A n), non-zero only for unclosed cells
-                    const double xs = L.x * double(a.cell_S[3 * c]) + L.y * double(a.cell_S[3 * c + 1]);
+                    const float2 cS = RL ? csl[s] : make_float2(a.cell_S[3 * c], a.cell_S[3 * c + 1]);
+                    const double xs = L.x * double(cS.x) + L.y * double(cS.y);
                     F[0] += L.gref_d * xs;
                     F[1] += L.href_d * xs;
                     float g1, h1, g2, h2;
@@ -590,7 +671,8 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 }
             } else {
                 if constexpr (sizeof(S) == 4) {
-                    const double xs = L.x * double(a.cell_S[3 * c]) + L.y * double(a.cell_S[3 * c + 1]);
+                    const float2 cS = RL ? csl[s] : make_float2(a.cell_S[3 * c], a.cell_S[3 * c + 1]);
+                    const double xs = L.x * double(cS.x) + L.y * double(cS.y);
                     F[0] += L.gref_d * xs;
                     const double feqn = bgk_devf(R.en, L.xf), feq = bgk_devf(R.e, L.xf);
                     const double fn = fcg + R.cm * (feqn - fcg);

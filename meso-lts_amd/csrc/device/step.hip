@@ -1407,6 +1407,8 @@ struct meso_hip_ctx {
         mp_maxr = 0;
     int p_chunks = 0, p_vchunk = 0;
     size_t lds_up = 0, lds_mp = 0;
+    bool upd_rl = false;      // k_patch_update stages face and cell records in LDS
+    bool upd_attr = false;    // its dynamic-LDS limit has been raised
     meso_patch::Item *up_items = nullptr, *mp_items = nullptr;
     meso_patch::Face *up_faces = nullptr, *mp_faces = nullptr;
     int *up_item_ptr = nullptr, *up_face_ptr = nullptr, *up_nown = nullptr, *mp_item_ptr = nullptr,
@@ -1744,7 +1746,18 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
     if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, 2, T>), s.nb, 0, a);
     if (c->n_int > 0) {
         LAUNCH(c, "cell_prep", (k_cell_prep<MODEL, T, TAB>), (c->n_int + WPB - 1) / WPB, a);
-        LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB>), c->n_up * c->p_chunks, c->lds_up, a);
+        if (c->upd_rl) {
+            if (!c->upd_attr && c->lds_up > 65536) {
+                HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_update<MODEL, T, VTU, TAB, true>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_up)));
+                c->upd_attr = true;
+            }
+            LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, true>), c->n_up * c->p_chunks, c->lds_up,
+                     a);
+        } else {
+            LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, false>), c->n_up * c->p_chunks, c->lds_up,
+                     a);
+        }
     }
     const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
     if (c->n_bcell > 0)
@@ -2236,13 +2249,24 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             if (kind == MESO_BC_SYMMETRY || kind == MESO_BC_SLIP_WALL) pown[m->face_cells[2 * f]] = 0;
         }
         auto rec = [&](int ipw, int mi, int mf) { return ipw > 1 ? size_t(mi) * si + size_t(mf) * sf : size_t(0); };
+        // The update kernel copies the face equilibria and cell constants of its
+        // patch to LDS as well; it may take up to half the LDS of a CU, so that
+        // two blocks share a CU
+        int lds_blk = 65536, lds_cu = 65536;
+        (void)hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+        (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device);
+        c->upd_rl = env_int("MESO_HIP_RECLDS", 1) != 0;
+        const size_t cap_up = c->upd_rl ? std::min<size_t>(size_t(lds_blk), size_t(std::max(lds_cu, lds_blk)) / 2) : cap;
         for (int P = env_int("MESO_HIP_PATCH_CELLS", 32); ok; P = P * 3 / 4) {
             ok = meso_patch::build_update_patches(mv, pown, P, ups, perr);
-            // [items | faces] | rows | values | face fluxes | q partials (k_patch_update)
+            // [items | faces] | rows | values | face fluxes | q partials | [face and cell records] (k_patch_update)
             c->lds_up = rec(IPWU, ups.max_items, ups.max_faces) + r16(4 * size_t(ups.max_rows)) +
                         r16(size_t(ND) * ups.max_rows * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
                         size_t(ups.max_owned) * 16;
-            if (c->lds_up <= cap || P <= 4) break;
+            if (c->upd_rl)
+                c->lds_up += storage == MESO_STORAGE_FP32 ? update_rec_bytes<float>(ups.max_faces, ups.max_owned)
+                                                          : update_rec_bytes<double>(ups.max_faces, ups.max_owned);
+            if (c->lds_up <= cap_up || P <= 4) break;
         }
         const int fcap = env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
         for (int F = std::min(fcap, 64); ok; F = F * 3 / 4) {
@@ -2255,7 +2279,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             c->lds_mp = std::max(a1, a2);
             if (c->lds_mp <= cap || F <= 4) break;
         }
-        if (ok && c->lds_up <= cap && c->lds_mp <= cap && ups.n_patch > 0 && mps.n_patch > 0) {
+        if (ok && c->lds_up <= cap_up && c->lds_mp <= cap && ups.n_patch > 0 && mps.n_patch > 0) {
             c->patches = true;
             c->p_vchunk = std::max(64, (env_int("MESO_HIP_PCHUNK", 2560) + 63) / 64 * 64);
             c->p_chunks = (nv + c->p_vchunk - 1) / c->p_vchunk;
