@@ -451,6 +451,63 @@ __device__ __forceinline__ T tab_entry(const KArgs &a, const EqParams &e, int t)
     else return T(expm1(a.ref.B * X * X - e.B * c * c));
 }
 
+// Flux moments of the Shakhov equilibrium along n for a tensor-product
+// velocity set, from 1D sums: with c = xi - u, every moment
+// sum_v w (xi.n) psi(xi) g_S(xi) is A times a polynomial in c of degree <= 6
+// against w_x w_y w_z E_x E_y E_z (E_a = exp(-B c_a^2)), so it is a
+// combination of the products Cx[a] Cy[b] Cz[c] of the central 1D moments
+// C_d[k] = sum_i w_d,i E_d,i c_d,i^k (k <= 6). Returns the D + 2 moments
+// psi = 1, xi_1..xi_D, |xi|^2 g + h (the order of Fb in k_face_eq).
+// g_S = g_M (1 + P1 c.q (|c|^2/RT - D - 2)),
+// h_S = g_M (hk + P2 c.q ((|c|^2/RT - D)(K + 3 - D) - 2K)) (cdugks_shakhov.cpp:198-211).
+template <int D>
+__device__ __forceinline__ void shakhov_flux_sep(const EqParams &e, const double (&C)[3][7], const double n[3], int K,
+                                                 double *out) {
+    const double kD = double(K + 3 - D);
+    auto M = [&](int a, int b, int c) { return C[0][a] * C[1][b] * C[2][c]; };
+    // <X^a Y^b Z^c (c.q) (iRT |c|^2 - s)>
+    auto Q = [&](int a, int b, int c, double r, double s) {
+        double t = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int a1 = a + (d == 0), b1 = b + (d == 1), c1 = c + (d == 2);
+            double sq = M(a1 + 2, b1, c1) + M(a1, b1 + 2, c1);
+            if (D == 3) sq += M(a1, b1, c1 + 2);
+            t += e.q[d] * (r * sq - s * M(a1, b1, c1));
+        }
+        return t;
+    };
+    auto G = [&](int a, int b, int c) { return M(a, b, c) + e.P1 * Q(a, b, c, e.iRT, double(D + 2)); };
+    auto H = [&](int a, int b, int c) {
+        return e.hk * M(a, b, c) + e.P2 * Q(a, b, c, kD * e.iRT, kD * double(D) + 2.0 * double(K));
+    };
+    double s0 = 0.0, uu = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        s0 += n[d] * e.u[d];
+        uu += e.u[d] * e.u[d];
+    }
+    // <(xi.n) X^a Y^b Z^c g_S / g_M-factor>, xi.n = s0 + n.c
+    auto SG = [&](int a, int b, int c) {
+        double t = s0 * G(a, b, c) + n[0] * G(a + 1, b, c) + n[1] * G(a, b + 1, c);
+        if (D == 3) t += n[2] * G(a, b, c + 1);
+        return t;
+    };
+    const double sg0 = SG(0, 0, 0);
+    double sgd[3] = {SG(1, 0, 0), SG(0, 1, 0), D == 3 ? SG(0, 0, 1) : 0.0};
+    out[0] = e.A * sg0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) out[1 + d] = e.A * (sgd[d] + e.u[d] * sg0);
+    double sh0 = s0 * H(0, 0, 0) + n[0] * H(1, 0, 0) + n[1] * H(0, 1, 0);
+    if (D == 3) sh0 += n[2] * H(0, 0, 1);
+    double e2 = SG(2, 0, 0) + SG(0, 2, 0);
+    if (D == 3) e2 += SG(0, 0, 2);
+    double ud = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) ud += e.u[d] * sgd[d];
+    out[D + 1] = e.A * (e2 + 2.0 * ud + uu * sg0 + sh0);
+}
+
 // (2) face macros, equilibrium constants, axis tables and flux moments of the
 // mixed value of interior faces; one wave per face
 template <int MODEL, int D, class T, bool TAB>
@@ -557,6 +614,45 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
         interior = a.face_bnd[f] < 0;
     }
     if (TAB) __syncthreads();
+    if (MODEL == CDUGKS_SHAKHOV && TAB && a.tab_w != nullptr) {
+        // tensor-product set with factorised weights: 1D central moments per
+        // axis (lanes over the axis entries), then the closed combination
+        if (act && interior) {
+            double C[3][7];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int k = 0; k < 7; ++k) C[d][k] = 0.0;
+            for (int t = lane; t < a.nt; t += 64) {
+                const int ax = a.tab_ax[t];
+                const double c = a.tab_x[t] - (ax == 0 ? e.u[0] : (ax == 1 ? e.u[1] : e.u[2]));
+                double m = a.tab_w[t] * exp(-e.B * c * c);
+#pragma unroll
+                for (int k = 0; k < 7; ++k) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        if (d == ax) C[d][k] += m;
+                    m *= c;
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < D; ++d)
+#pragma unroll
+                for (int k = 0; k < 7; ++k) C[d][k] = seg_sum<64>(C[d][k]);
+            if (D == 2) {
+                C[2][0] = 1.0;  // the third axis is the single abscissa 0 with weight 1
+#pragma unroll
+                for (int k = 1; k < 7; ++k) C[2][k] = 0.0;
+            }
+            if (lane == 0) {
+                double full[NFL];
+                shakhov_flux_sep<D>(e, C, n, a.s.K, full);
+                const double mix = e.mix;
+                for (int k = 0; k < NFL; ++k) a.face_flux[size_t(f) * NFL + k] = (1.0 - mix) * Fb[k] + mix * full[k];
+            }
+        }
+        return;
+    }
     if (act && interior) {
         const int nv = a.s.nv;
         for (int v = lane; v < nv; v += 64) {
@@ -1392,7 +1488,7 @@ struct meso_hip_ctx {
     float2 *fref_f2 = nullptr;
     int nt = 0;
     int *vtab = nullptr, *tab_ax = nullptr;
-    double *tab_x = nullptr;
+    double *tab_x = nullptr, *tab_w = nullptr;
     void *face_tab = nullptr;
     int *cell_list = nullptr, *st_id = nullptr, *fs_id = nullptr;
     double *st_a = nullptr, *fs_a = nullptr;
@@ -1545,6 +1641,7 @@ KArgs make_args(meso_hip_ctx *c) {
     a.vtab = c->vtab;
     a.tab_x = c->tab_x;
     a.tab_ax = c->tab_ax;
+    a.tab_w = c->tab_w;
     a.face_tab = c->face_tab;
     a.cell_list = c->cell_list;
     a.st_id = c->st_id;
@@ -2105,7 +2202,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         t_vf[v] = make_float4(float(x[0]), float(x[1]), float(x[2]), float(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]));
     }
     std::vector<int> t_vtab, t_tax;
-    std::vector<double> t_tx;
+    std::vector<double> t_tx, t_tw;
     if (p->model == MESO_MODEL_CDUGKS_SHAKHOV && env_int("MESO_HIP_NO_TABLES", 0) == 0) {
         // unique abscissae per axis; tensor when every (i,j,k) combination occurs once
         std::vector<std::vector<double>> axes(D);
@@ -2148,6 +2245,34 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
                         t_tax.push_back(dd);
                     }
                 c->nt = int(t_tx.size());
+                // 1D weights when w(v) is the product of per-axis weights (to
+                // 1e-13): taken along the axis lines through the heaviest velocity
+                int vm = 0;
+                for (int v = 1; v < nv; ++v)
+                    if (d->w[v] > d->w[vm]) vm = v;
+                std::vector<int> im(D);
+                for (int dd = 0; dd < D; ++dd) im[dd] = (t_vtab[vm] >> (10 * dd)) & 1023;
+                std::vector<double> tw(c->nt, 0.0);
+                bool fact = d->w[vm] > 0.0;
+                for (int v = 0; v < nv && fact; ++v) {
+                    int same = 0, axis = -1;
+                    for (int dd = 0; dd < D; ++dd) {
+                        if (((t_vtab[v] >> (10 * dd)) & 1023) == im[dd]) ++same;
+                        else axis = dd;
+                    }
+                    if (same == D) {
+                        for (int dd = 0; dd < D; ++dd) tw[im[dd]] = dd == 0 ? d->w[vm] : 1.0;
+                    } else if (same == D - 1) {
+                        const int tt = (t_vtab[v] >> (10 * axis)) & 1023;
+                        tw[tt] = axis == 0 ? d->w[v] : d->w[v] / d->w[vm];
+                    }
+                }
+                for (int v = 0; v < nv && fact; ++v) {
+                    double prod = 1.0;
+                    for (int dd = 0; dd < D; ++dd) prod *= tw[(t_vtab[v] >> (10 * dd)) & 1023];
+                    fact = std::fabs(prod - d->w[v]) <= 1e-13 * std::fabs(d->w[v]) + 1e-300;
+                }
+                if (fact && env_int("MESO_HIP_SEP_FLUX", 1) != 0) t_tw = tw;
             }
         }
     }
@@ -2366,6 +2491,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         UP(vtab, t_vtab.data(), t_vtab.size());
         UP(tab_x, t_tx.data(), t_tx.size());
         UP(tab_ax, t_tax.data(), t_tax.size());
+        if (!t_tw.empty()) UP(tab_w, t_tw.data(), t_tw.size());
     }
     UP(face_bnd, c->face_bnd_h.data(), nf);
     UP(bnd_face, c->bnd_face_h.data(), c->bnd_face_h.size());

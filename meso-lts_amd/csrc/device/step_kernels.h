@@ -72,6 +72,7 @@ struct KArgs {
     const int *vtab;            // [nv] packed offsets of a velocity's axis entries
     const double *tab_x;        // [nt] abscissa of each entry
     const int *tab_ax;          // [nt] axis of each entry
+    const double *tab_w;        // [nt] 1D weight of each entry (w(xi) = prod of its axis weights), or null
     void *face_tab;             // [nf*nt] per-face axis factors (storage type)
     // gather stencils: group = centre cell + its LSQ neighbours (padded with
     // the centre and zero weights); cells: groups of the cell and of the cell
