@@ -1,4 +1,4 @@
-// LDS-staged patch kernels of the 2D step (included by step.hip inside its
+// LDS-staged patch kernels of the step (included by step.hip inside its
 // anonymous namespace; they use its helpers).
 //
 // A block owns one spatial patch (csrc/host/patches.h) and one chunk of the
@@ -24,25 +24,25 @@
 constexpr int BLOCKP = 512;  // threads of a patch block
 
 // Per-cell constants of the update (written by k_cell_prep for interior cells).
-template <class S>
+template <class S, int D>
 struct CellRec;
-template <>
-struct CellRec<float> {
+template <int D>
+struct CellRec<float, D> {
     EqF en, e;                        // equilibria at t_n and t_{n+1}, deviation form
     double cm, C, ai, an, Cs, dtv;    // see k_patch_update
-    double u0, u1;                    // u^{n+1}
+    double u[D];                      // u^{n+1}
 };
-template <>
-struct CellRec<double> {
+template <int D>
+struct CellRec<double, D> {
     EqParams en, e;
     double cm, C, itau, itaun, Cs, dtv;
-    double u0, u1;
+    double u[D];
 };
 
 // ------------------------------------------------------------------- prep
 // Per cell of the update patches: conservative macro update (as k_cell_update
 // does for its cells), the constants of the f update, the cell's axis tables.
-template <int MODEL, class S, bool TAB>
+template <int MODEL, class S, bool TAB, int D>
 __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
     if (*a.halt) return;
     constexpr int WPB = BLOCK / 64;
@@ -50,22 +50,22 @@ __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
     const int li = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + int(threadIdx.x / 64));
     if (li >= a.n_int_cells) return;
     const int c = a.cell_list[li];
-    CellStep<MODEL, 2> cs;
-    const bool ok = cell_macros<MODEL, 2, S>(a, c, cs);
+    CellStep<MODEL, D> cs;
+    const bool ok = cell_macros<MODEL, D, S>(a, c, cs);
     if (lane == 0) {
         a.rho_out[c] = cs.rho;
         for (int i = 0; i < 3; ++i) a.u_out[3 * c + i] = cs.u[i];
         if (MODEL == CDUGKS_SHAKHOV) a.T_out[c] = cs.T;
         for (int k = 0; k < 5; ++k) a.fm[5 * c + k] = 0.0;
         a.fm[5 * c] = cs.phi[0];
-        for (int i = 0; i < 2; ++i) a.fm[5 * c + 1 + i] = cs.phi[1 + i];
-        if (MODEL == CDUGKS_SHAKHOV) a.fm[5 * c + 4] = cs.phi[3];
+        for (int i = 0; i < D; ++i) a.fm[5 * c + 1 + i] = cs.phi[1 + i];
+        if (MODEL == CDUGKS_SHAKHOV) a.fm[5 * c + 4] = cs.phi[D + 1];
         if (!ok) {
             atomicAdd(a.bad_count, 1);
             atomicMin(a.first_bad, c);
             atomicMin(a.bad_step, a.step_index);
         }
-        CellRec<S> r;
+        CellRec<S, D> r;
         if constexpr (sizeof(S) == 4) {
             r.en = to_eqf(cs.en);
             r.e = to_eqf(cs.e);
@@ -85,9 +85,8 @@ __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
         r.C = cs.C;
         r.Cs = cs.Cs;
         r.dtv = cs.dt_v;
-        r.u0 = cs.u[0];
-        r.u1 = cs.u[1];
-        reinterpret_cast<CellRec<S> *>(a.crec)[c] = r;
+        for (int i = 0; i < D; ++i) r.u[i] = cs.u[i];
+        reinterpret_cast<CellRec<S, D> *>(a.crec)[c] = r;
     }
     if (TAB) {
         S *dst = reinterpret_cast<S *>(a.cell_tab) + size_t(c) * 2 * a.nt;
@@ -98,17 +97,18 @@ __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
     }
 }
 
-// LSQ gradient (gx, gy) of one item from the staged rows of distribution d:
+// LSQ gradient g[0..D-1] of one item from the staged rows of distribution d:
 // sum_j a_j (f_j - f_c) (mesh_numerical.cpp:24-49).
-template <class S, int VT>
-__device__ __forceinline__ void item_grad(const meso_patch::Item *it, const S *val_d, int vl, double fc, double &gx,
-                                          double &gy) {
-    gx = gy = 0.0;
+template <class S, int VT, int D>
+__device__ __forceinline__ void item_grad(const meso_patch::ItemT<D> *it, const S *val_d, int vl, double fc,
+                                          double *g) {
 #pragma unroll
-    for (int j = 0; j < meso_patch::MAXN; ++j) {
+    for (int k = 0; k < D; ++k) g[k] = 0.0;
+#pragma unroll
+    for (int j = 0; j < meso_patch::Lim<D>::MAXN; ++j) {
         const double df = double(val_d[it->nb[j] * VT + vl]) - fc;
-        gx += it->a[j][0] * df;
-        gy += it->a[j][1] * df;
+#pragma unroll
+        for (int k = 0; k < D; ++k) g[k] += it->a[j][k] * df;
     }
 }
 
@@ -123,18 +123,18 @@ __device__ __forceinline__ void stage_copy(const R *src, int n, R *dst) {
 // Patch records: with one item per wave (IPW == 1) every record a wave reads
 // is wave-uniform and is read from HBM through the scalar cache; with several
 // items per wave the records are staged in LDS.
-template <int IPW>
+template <int IPW, int D>
 struct Records {
-    const meso_patch::Item *items;
-    const meso_patch::Face *faces;
-    __device__ __forceinline__ const meso_patch::Item *item(int s) const { return items + s; }
-    __device__ __forceinline__ const meso_patch::Face *face(int lf) const { return faces + lf; }
+    const meso_patch::ItemT<D> *items;
+    const meso_patch::FaceT<D> *faces;
+    __device__ __forceinline__ const meso_patch::ItemT<D> *item(int s) const { return items + s; }
+    __device__ __forceinline__ const meso_patch::FaceT<D> *face(int lf) const { return faces + lf; }
 };
 
 // Bytes of LDS the records take (0 when they are read from HBM).
-template <int IPW>
+template <int IPW, int D>
 __host__ __device__ constexpr size_t record_bytes(int maxi, int maxf) {
-    return IPW > 1 ? size_t(maxi) * sizeof(meso_patch::Item) + size_t(maxf) * sizeof(meso_patch::Face) : 0;
+    return IPW > 1 ? size_t(maxi) * sizeof(meso_patch::ItemT<D>) + size_t(maxf) * sizeof(meso_patch::FaceT<D>) : 0;
 }
 
 // Stage the rows of one velocity tile: all loads of a batch are issued before
@@ -166,14 +166,15 @@ __device__ __forceinline__ void stage_values(const int *rows, int nr, const S *f
 
 // Lane velocity data of one tile.
 struct LaneVel {
-    double x, y, w;   // abscissae and weight (FP64)
-    float xf[3];      // FP32 abscissae (z = 0)
+    double x, y, z, w;  // abscissae and weight (FP64); z = 0 in 2D
+    float xf[3];        // FP32 abscissae
     int vt;           // packed axis-table offsets
     float gref;       // FP32 g_ref
     double gref_d, href_d;
     bool ok;          // v < nv
 };
 
+template <int D>
 __device__ __forceinline__ LaneVel lane_vel(const KArgs &a, int v) {
     LaneVel L;
     L.ok = v < a.s.nv;
@@ -181,11 +182,12 @@ __device__ __forceinline__ LaneVel lane_vel(const KArgs &a, int v) {
     const double4 X = a.vel_d[vv];
     L.x = X.x;
     L.y = X.y;
+    L.z = D == 3 ? X.z : 0.0;
     L.w = L.ok ? X.w : 0.0;
     const float4 xf = a.vel_f[vv];
     L.xf[0] = xf.x;
     L.xf[1] = xf.y;
-    L.xf[2] = 0.0f;
+    L.xf[2] = D == 3 ? xf.z : 0.0f;
     L.vt = a.nt > 0 ? a.vtab[vv] : 0;
     L.gref = a.fref_f2[vv].x;
     const double2 fr = a.fref_d2[vv];
@@ -196,17 +198,17 @@ __device__ __forceinline__ LaneVel lane_vel(const KArgs &a, int v) {
 
 // Equilibrium pair of face f at the lane velocity, in the storage form
 // (deviation for FP32, full value for FP64), as FP64.
-template <int MODEL, class S, bool TAB>
+template <int MODEL, class S, bool TAB, int D>
 __device__ __forceinline__ void face_eq_lane(const KArgs &a, int f, const LaneVel &L, double &gs, double &hs) {
     if constexpr (sizeof(S) == 4) {
         const EqF &e = a.face_eqf[f];
         float g, h = 0.0f;
         if (MODEL == CDUGKS_SHAKHOV) {
             if (TAB)
-                shakhov_tab_f(e, reinterpret_cast<const float *>(a.face_tab) + size_t(f) * a.nt, L.vt, L.xf, 2, a.s.K,
+                shakhov_tab_f(e, reinterpret_cast<const float *>(a.face_tab) + size_t(f) * a.nt, L.vt, L.xf, D, a.s.K,
                               L.gref, g, h);
             else
-                shakhov_devf(e, L.xf, 2, a.s.K, L.gref, g, h);
+                shakhov_devf(e, L.xf, D, a.s.K, L.gref, g, h);
         } else {
             g = bgk_devf(e, L.xf);
         }
@@ -214,13 +216,13 @@ __device__ __forceinline__ void face_eq_lane(const KArgs &a, int f, const LaneVe
         hs = h;
     } else {
         const EqParams &e = a.face_eq[f];
-        const double x[3] = {L.x, L.y, 0.0};
+        const double x[3] = {L.x, L.y, L.z};
         if (MODEL == CDUGKS_SHAKHOV) {
             if (TAB)
-                shakhov_tab_d(e, reinterpret_cast<const double *>(a.face_tab) + size_t(f) * a.nt, L.vt, x, 2, a.s.K,
+                shakhov_tab_d(e, reinterpret_cast<const double *>(a.face_tab) + size_t(f) * a.nt, L.vt, x, D, a.s.K,
                               gs, hs);
             else
-                shakhov_eq<double>(e, x, 2, a.s.K, gs, hs);
+                shakhov_eq<double>(e, x, D, a.s.K, gs, hs);
         } else {
             gs = bgk_eq<double>(e, x);
             hs = 0.0;
@@ -242,37 +244,65 @@ struct UpdateRecs<double> {
     __device__ static EqParams face_eq(const KArgs &a, int f) { return a.face_eq[f]; }
 };
 
+// f_ref flux factor sum(+-A n) of a cell as the update kernel keeps it.
+struct CellS3 {
+    float x, y, z, pad;
+};
+template <int D>
+using CellS = typename std::conditional<D == 2, float2, CellS3>::type;
+
+template <int D>
+__device__ __forceinline__ CellS<D> cell_s(const float *p) {
+    CellS<D> r;
+    r.x = p[0];
+    r.y = p[1];
+    if constexpr (D == 3) {
+        r.z = p[2];
+        r.pad = 0.0f;
+    }
+    return r;
+}
+
+// xi . sum(+-A n) of a cell
+template <int D>
+__device__ __forceinline__ double cell_xs(const CellS<D> &s, const double *xv) {
+    double r = xv[0] * double(s.x) + xv[1] * double(s.y);
+    if constexpr (D == 3) r += xv[2] * double(s.z);
+    return r;
+}
+
 // Bytes of LDS the face and cell records of k_patch_update take with RL.
-template <class S>
+template <class S, int D>
 __host__ __device__ constexpr size_t update_rec_bytes(int maxf, int maxo) {
-    return size_t(maxf) * (sizeof(typename UpdateRecs<S>::Eq) + 8) + size_t(maxo) * (sizeof(CellRec<S>) + 8);
+    return size_t(maxf) * (sizeof(typename UpdateRecs<S>::Eq) + 8) +
+           size_t(maxo) * (sizeof(CellRec<S, D>) + sizeof(CellS<D>));
 }
 
 // face_eq_lane with the face's equilibrium record e at hand.
-template <int MODEL, class S, bool TAB>
+template <int MODEL, class S, bool TAB, int D>
 __device__ __forceinline__ void face_eq_rec(const KArgs &a, const typename UpdateRecs<S>::Eq &e, int f,
                                             const LaneVel &L, double &gs, double &hs) {
     if constexpr (sizeof(S) == 4) {
         float g, h = 0.0f;
         if (MODEL == CDUGKS_SHAKHOV) {
             if (TAB)
-                shakhov_tab_f(e, reinterpret_cast<const float *>(a.face_tab) + size_t(f) * a.nt, L.vt, L.xf, 2, a.s.K,
+                shakhov_tab_f(e, reinterpret_cast<const float *>(a.face_tab) + size_t(f) * a.nt, L.vt, L.xf, D, a.s.K,
                               L.gref, g, h);
             else
-                shakhov_devf(e, L.xf, 2, a.s.K, L.gref, g, h);
+                shakhov_devf(e, L.xf, D, a.s.K, L.gref, g, h);
         } else {
             g = bgk_devf(e, L.xf);
         }
         gs = g;
         hs = h;
     } else {
-        const double x[3] = {L.x, L.y, 0.0};
+        const double x[3] = {L.x, L.y, L.z};
         if (MODEL == CDUGKS_SHAKHOV) {
             if (TAB)
-                shakhov_tab_d(e, reinterpret_cast<const double *>(a.face_tab) + size_t(f) * a.nt, L.vt, x, 2, a.s.K,
+                shakhov_tab_d(e, reinterpret_cast<const double *>(a.face_tab) + size_t(f) * a.nt, L.vt, x, D, a.s.K,
                               gs, hs);
             else
-                shakhov_eq<double>(e, x, 2, a.s.K, gs, hs);
+                shakhov_eq<double>(e, x, D, a.s.K, gs, hs);
         } else {
             gs = bgk_eq<double>(e, x);
             hs = 0.0;
@@ -285,7 +315,7 @@ __device__ __forceinline__ void face_eq_rec(const KArgs &a, const typename Updat
 // equilibrium (FP64), the outgoing half keeps fm; on the dead-band a wall takes
 // the average (cdugks_shakhov.cpp:364-607). Mirror boundaries (symmetry,
 // slip-wall) never reach this kernel.
-template <int MODEL, class S>
+template <int MODEL, class S, int D>
 __device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn, const LaneVel &L, double gm,
                                                double hm, double &g, double &h) {
     g = gm;
@@ -295,9 +325,9 @@ __device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn,
     const BndParams &P = a.bnd[bi];
     const double kn = -sn;  // xi . n1, n1 = -n0 points into the domain
     if (P.kind == BC_INTERIOR || kn < -VSMALL) return;
-    const double x[3] = {L.x, L.y, 0.0};
+    const double x[3] = {L.x, L.y, L.z};
     double gi, hi = 0.0;
-    if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<double>(P.in, x, 2, a.s.K, gi, hi);
+    if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<double>(P.in, x, D, a.s.K, gi, hi);
     else gi = bgk_eq<double>(P.in, x);
     if constexpr (sizeof(S) == 4) {
         gi -= L.gref_d;
@@ -315,7 +345,7 @@ __device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn,
 // ------------------------------------------------------------- moments
 // Raw moments of f-bar+ per face of the moment patches, summed per velocity
 // chunk into part_f[face][chunk][col] (the layout k_face_eq reads).
-template <int MODEL, class S, int VT>
+template <int MODEL, class S, int VT, int D>
 __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.bad_count > 0) {
@@ -324,11 +354,15 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     }
     constexpr bool SH = MODEL == CDUGKS_SHAKHOV;
     constexpr int ND = SH ? 2 : 1;
-    constexpr int NCOL = ncol_face<MODEL, 2>();
+    constexpr int NCOL = ncol_face<MODEL, D>();
     constexpr int IPW = 64 / VT;
     constexpr int NW = BLOCKP / 64;
     constexpr int NSLOT = NW * IPW;
     constexpr int VTP = VT + 1;  // odd row stride: conflict-free reads with lane = face
+    using Item = meso_patch::ItemT<D>;
+    using Face = meso_patch::FaceT<D>;
+    const Item *mp_items = reinterpret_cast<const Item *>(a.mp_items);
+    const Face *mp_faces = reinterpret_cast<const Face *>(a.mp_faces);
     const int np = a.n_mp;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int chunk = b / np, p = b - chunk * np;
@@ -338,18 +372,18 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     const int maxi = a.mp_maxi, maxf = a.mp_maxf, maxr = a.mp_maxr;
     // LDS: [items | faces] | rows | tile velocities | values | f-bar+ rows
     unsigned char *q = lds_raw;
-    Records<IPW> rec;
+    Records<IPW, D> rec;
     if constexpr (IPW > 1) {
-        auto *li = reinterpret_cast<meso_patch::Item *>(q);
-        auto *lf = reinterpret_cast<meso_patch::Face *>(li + maxi);
-        stage_copy(a.mp_items + i0, ni, li);
-        stage_copy(a.mp_faces + f0, nfp, lf);
+        auto *li = reinterpret_cast<Item *>(q);
+        auto *lf = reinterpret_cast<Face *>(li + maxi);
+        stage_copy(mp_items + i0, ni, li);
+        stage_copy(mp_faces + f0, nfp, lf);
         rec.items = li;
         rec.faces = lf;
-        q += record_bytes<IPW>(maxi, maxf);
+        q += record_bytes<IPW, D>(maxi, maxf);
     } else {
-        rec.items = a.mp_items + i0;
-        rec.faces = a.mp_faces + f0;
+        rec.items = mp_items + i0;
+        rec.faces = mp_faces + f0;
     }
     int *rows = reinterpret_cast<int *>(q);
     q += (size_t(maxr) * 4 + 31) & ~size_t(31);
@@ -388,31 +422,39 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         __syncthreads();
         // B: gradients and f-bar+ of the patch faces
         for (int s = slot; s < ni; s += NSLOT) {
-            const meso_patch::Item *it = rec.item(s);
+            const Item *it = rec.item(s);
             const int nface = it->nface;
-            double fc[2], gx[2], gy[2];
+            double fc[2], g[2][D];
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 fc[d] = double(val[(d * maxr + s) * VT + vl]);
-                gx[d] = gy[d] = 0.0;
-                if (a.s.grad) item_grad<S, VT>(it, val + d * maxr * VT, vl, fc[d], gx[d], gy[d]);
-            }
 #pragma unroll
-            for (int k = 0; k < meso_patch::MAXF; ++k) {
+                for (int k = 0; k < D; ++k) g[d][k] = 0.0;
+                if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d]);
+            }
+            const double xv[3] = {X.x, X.y, X.z};
+#pragma unroll
+            for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
                 if (k >= nface) continue;
                 const int e = it->face[k];
                 const int lf = e & 0xffff, mode = e >> 16;
-                const meso_patch::Face *F = rec.face(lf);
-                const double sn = F->n[0] * X.x + F->n[1] * X.y;  // + n_z xi_z, both 0 in 2D
+                const Face *F = rec.face(lf);
+                double sn = 0.0;  // in 2D: + n_z xi_z, both 0
+#pragma unroll
+                for (int i = 0; i < D; ++i) sn += F->n[i] * xv[i];
                 const bool up = mode == meso_patch::SIDE_BOUNDARY ||
                                 (mode == meso_patch::SIDE_OWNER ? sn >= VSMALL : sn < -VSMALL);
                 const bool dead = !(sn >= VSMALL) && !(sn < -VSMALL);
                 const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
-                const double ex = F->d[side][0] - X.x * hdt, ey = F->d[side][1] - X.y * hdt;
+                double ev[D];
+#pragma unroll
+                for (int i = 0; i < D; ++i) ev[i] = F->d[side][i] - xv[i] * hdt;
                 if (!ok) continue;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                    const double Xd = ex * gx[d] + ey * gy[d];
+                    double Xd = 0.0;
+#pragma unroll
+                    for (int i = 0; i < D; ++i) Xd += ev[i] * g[d][i];
                     S *dst = &fb[(d * maxf + lf) * VTP + vl];
                     if (mode == meso_patch::SIDE_BOUNDARY) {
                         *dst = S(dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd) : fc[d] + Xd);
@@ -430,21 +472,33 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
             for (int vv = vs; vv < VT; vv += nvs) {
                 const double4 Y = vtile[vv];
                 const double wg = Y.w * double(fb[my_face * VTP + vv]);
-                const double wgx = wg * Y.x, wgy = wg * Y.y;
-                acc[Col<2>::M0] += wg;
-                acc[Col<2>::M1] += wgx;
-                acc[Col<2>::M1 + 1] += wgy;
-                acc[Col<2>::M2] += wgx * Y.x;
-                acc[Col<2>::M2 + 1] += wgx * Y.y;
-                acc[Col<2>::M2 + 2] += wgy * Y.y;
+                const double wgx = wg * Y.x, wgy = wg * Y.y, wgz = wg * Y.z;
+                acc[Col<D>::M0] += wg;
+                acc[Col<D>::M1] += wgx;
+                acc[Col<D>::M1 + 1] += wgy;
+                if (D == 3) acc[Col<D>::M1 + 2] += wgz;
+                if (D == 2) {
+                    acc[Col<D>::M2] += wgx * Y.x;
+                    acc[Col<D>::M2 + 1] += wgx * Y.y;
+                    acc[Col<D>::M2 + 2] += wgy * Y.y;
+                } else {
+                    acc[Col<D>::M2] += wgx * Y.x;
+                    acc[Col<D>::M2 + 1] += wgx * Y.y;
+                    acc[Col<D>::M2 + 2] += wgx * Y.z;
+                    acc[Col<D>::M2 + 3] += wgy * Y.y;
+                    acc[Col<D>::M2 + 4] += wgy * Y.z;
+                    acc[Col<D>::M2 + 5] += wgz * Y.z;
+                }
                 if (SH) {
                     const double wh = Y.w * double(fb[(maxf + my_face) * VTP + vv]);
                     const double wgk = wg * (Y.x * Y.x + Y.y * Y.y + Y.z * Y.z);
-                    acc[Col<2>::M3] += wgk * Y.x;
-                    acc[Col<2>::M3 + 1] += wgk * Y.y;
-                    acc[Col<2>::H0] += wh;
-                    acc[Col<2>::H1] += wh * Y.x;
-                    acc[Col<2>::H1 + 1] += wh * Y.y;
+                    acc[Col<D>::M3] += wgk * Y.x;
+                    acc[Col<D>::M3 + 1] += wgk * Y.y;
+                    if (D == 3) acc[Col<D>::M3 + 2] += wgk * Y.z;
+                    acc[Col<D>::H0] += wh;
+                    acc[Col<D>::H1] += wh * Y.x;
+                    acc[Col<D>::H1 + 1] += wh * Y.y;
+                    if (D == 3) acc[Col<D>::H1 + 2] += wh * Y.z;
                 }
             }
         }
@@ -460,7 +514,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         const int lf = r / NCOL, k = r - lf * NCOL;
         double sum = 0.0;
         for (int qq = 0; qq < nvs; ++qq) sum += red[(size_t(qq) * FG + lf) * NCOL + k];
-        const int f = a.mp_faces[f0 + lf].face;
+        const int f = mp_faces[f0 + lf].face;
         a.part_f[(size_t(f) * a.s.ntile_f + chunk) * NCOL + k] = sum;
     }
 }
@@ -471,7 +525,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
 // the patch faces and the update constants of the owned cells are copied to
 // LDS once per block; otherwise every tile re-reads them per lane from HBM
 // (L2), which costs address arithmetic and L2 latency in every tile.
-template <int MODEL, class S, int VT, bool TAB, bool RL>
+template <int MODEL, class S, int VT, bool TAB, bool RL, int D>
 __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.halt) return;
@@ -480,6 +534,10 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     constexpr int IPW = 64 / VT;
     constexpr int NW = BLOCKP / 64;
     constexpr int NSLOT = NW * IPW;
+    using Item = meso_patch::ItemT<D>;
+    using Face = meso_patch::FaceT<D>;
+    const Item *up_items = reinterpret_cast<const Item *>(a.up_items);
+    const Face *up_faces = reinterpret_cast<const Face *>(a.up_faces);
     const int np = a.n_up;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int chunk = b / np, p = b - chunk * np;
@@ -489,8 +547,8 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     const int nown = a.up_nown[p];
     const int maxi = a.up_maxi, maxf = a.up_maxf, maxr = a.up_maxr;
     // LDS: items | faces | rows | values | face fluxes | q partials
-    auto *items = reinterpret_cast<meso_patch::Item *>(lds_raw);
-    auto *faces = reinterpret_cast<meso_patch::Face *>(items + maxi);
+    auto *items = reinterpret_cast<Item *>(lds_raw);
+    auto *faces = reinterpret_cast<Face *>(items + maxi);
     int *rows = reinterpret_cast<int *>(faces + maxf);
     S *val = reinterpret_cast<S *>(lds_raw + ((reinterpret_cast<unsigned char *>(rows + maxr) - lds_raw + 15) & ~15));
     S *flx = val + size_t(ND) * maxr * VT;
@@ -499,10 +557,10 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     // with RL: the face equilibria and mix weights of the patch faces and the
     // update constants of the owned cells, copied once per block
     using EqS = typename UpdateRecs<S>::Eq;
-    EqS *feq = reinterpret_cast<EqS *>(qacc + 2 * a.up_maxo);
+    EqS *feq = reinterpret_cast<EqS *>(qacc + D * a.up_maxo);
     double *fmix = reinterpret_cast<double *>(feq + maxf);
-    CellRec<S> *crl = reinterpret_cast<CellRec<S> *>(fmix + maxf);
-    float2 *csl = reinterpret_cast<float2 *>(crl + a.up_maxo);
+    CellRec<S, D> *crl = reinterpret_cast<CellRec<S, D> *>(fmix + maxf);
+    CellS<D> *csl = reinterpret_cast<CellS<D> *>(crl + a.up_maxo);
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int sub = lane / VT, vl = lane % VT;
     const int slot = wave * IPW + sub;
@@ -510,20 +568,20 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     const double hdt = a.s.half_dt;
     const S *fin_g = (const S *)a.fin_g, *fin_h = (const S *)a.fin_h;
     S *fout_g = (S *)a.fout_g, *fout_h = (S *)a.fout_h;
-    stage_copy(a.up_items + i0, ni, items);
-    stage_copy(a.up_faces + f0, nfp, faces);
+    stage_copy(up_items + i0, ni, items);
+    stage_copy(up_faces + f0, nfp, faces);
     for (int r = threadIdx.x; r < nr; r += BLOCKP) rows[r] = a.up_rows[r0 + r];
-    for (int r = threadIdx.x; r < 2 * nown; r += BLOCKP) qacc[r] = 0.0;
+    for (int r = threadIdx.x; r < D * nown; r += BLOCKP) qacc[r] = 0.0;
     if constexpr (RL) {
         for (int lf = threadIdx.x; lf < nfp; lf += BLOCKP) {
-            const int f = a.up_faces[f0 + lf].face;
+            const int f = up_faces[f0 + lf].face;
             feq[lf] = UpdateRecs<S>::face_eq(a, f);
             fmix[lf] = a.face_eq[f].mix;
         }
         for (int r = threadIdx.x; r < nown; r += BLOCKP) {
-            const int c = a.up_items[i0 + r].cell;
-            crl[r] = reinterpret_cast<const CellRec<S> *>(a.crec)[c];
-            csl[r] = make_float2(a.cell_S[3 * c], a.cell_S[3 * c + 1]);
+            const int c = up_items[i0 + r].cell;
+            crl[r] = reinterpret_cast<const CellRec<S, D> *>(a.crec)[c];
+            csl[r] = cell_s<D>(a.cell_S + 3 * c);
         }
     }
     const int ntiles = (a.s.nv + VT - 1) / VT;
@@ -531,28 +589,32 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     __syncthreads();
     for (int t = t0; t < t1; ++t) {
         const int v = t * VT + vl;
-        const LaneVel L = lane_vel(a, v);
+        const LaneVel L = lane_vel<D>(a, v);
+        const double xv[3] = {L.x, L.y, L.z};
         // A: stage values
         stage_values<ND, S, VT>(rows, nr, fin_g, fin_h, nvp, v, L.ok, slot, vl, val, maxr);
         __syncthreads();
         // B: gradients; flux (xi.n) A f_face of each face on its upwind side's lanes
         for (int s = slot; s < ni; s += NSLOT) {
-            const meso_patch::Item *it = items + s;
+            const Item *it = items + s;
             const int nface = it->nface;
-            double fc[2], gx[2], gy[2];
+            double fc[2], g[2][D];
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 fc[d] = double(val[(d * maxr + s) * VT + vl]);
-                gx[d] = gy[d] = 0.0;
-                if (a.s.grad) item_grad<S, VT>(it, val + d * maxr * VT, vl, fc[d], gx[d], gy[d]);
+#pragma unroll
+                for (int k = 0; k < D; ++k) g[d][k] = 0.0;
+                if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d]);
             }
 #pragma unroll
-            for (int k = 0; k < meso_patch::MAXF; ++k) {
+            for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
                 if (k >= nface) continue;
                 const int e = it->face[k];
                 const int lf = e & 0xffff, mode = e >> 16;
-                const meso_patch::Face *F = faces + lf;
-                const double sn = F->n[0] * L.x + F->n[1] * L.y;
+                const Face *F = faces + lf;
+                double sn = 0.0;
+#pragma unroll
+                for (int i = 0; i < D; ++i) sn += F->n[i] * xv[i];
                 const bool bnd = mode == meso_patch::SIDE_BOUNDARY;
                 const bool own = mode == meso_patch::SIDE_OWNER;
                 const bool dead = !(sn >= VSMALL) && !(sn < -VSMALL);
@@ -563,26 +625,30 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                     double gs, hs = 0.0;
                     double mix;
                     if constexpr (RL) {
-                        face_eq_rec<MODEL, S, TAB>(a, feq[lf], f, L, gs, hs);
+                        face_eq_rec<MODEL, S, TAB, D>(a, feq[lf], f, L, gs, hs);
                         mix = fmix[lf];
                     } else {
-                        face_eq_lane<MODEL, S, TAB>(a, f, L, gs, hs);
+                        face_eq_lane<MODEL, S, TAB, D>(a, f, L, gs, hs);
                         mix = a.face_eq[f].mix;
                     }
                     const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
-                    const double ex = F->d[side][0] - L.x * hdt, ey = F->d[side][1] - L.y * hdt;
+                    double ev[D];
+#pragma unroll
+                    for (int i = 0; i < D; ++i) ev[i] = F->d[side][i] - xv[i] * hdt;
                     const double knA = sn * F->area;
                     double fm[2] = {0.0, 0.0};
 #pragma unroll
                     for (int d = 0; d < ND; ++d) {
-                        const double Xd = ex * gx[d] + ey * gy[d];
+                        double Xd = 0.0;
+#pragma unroll
+                        for (int i = 0; i < D; ++i) Xd += ev[i] * g[d][i];
                         // a boundary face's dead-band value 0.5 (f + f + X + X) is f + X
                         const double fbar = fc[d] + Xd;
                         const double eq = d == 0 ? gs : hs;
                         if constexpr (sizeof(S) == 8) fm[d] = (1.0 - mix) * fbar + mix * eq;
                         else fm[d] = fbar + mix * (eq - fbar);
                     }
-                    if (bnd) bnd_value_lane<MODEL, S>(a, F->face, sn, L, fm[0], fm[1], fm[0], fm[1]);
+                    if (bnd) bnd_value_lane<MODEL, S, D>(a, F->face, sn, L, fm[0], fm[1], fm[0], fm[1]);
 #pragma unroll
                     for (int d = 0; d < ND; ++d) flx[(d * maxf + lf) * VT + vl] = S(knA * fm[d]);
                 } else if (own && dead) {
@@ -594,12 +660,12 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
         __syncthreads();
         // C: owned cells
         for (int s = slot; s < nown; s += NSLOT) {
-            const meso_patch::Item *it = items + s;
+            const Item *it = items + s;
             const int c = it->cell;
             const int nface = it->nface;
             double F[2] = {0.0, 0.0};
 #pragma unroll
-            for (int k = 0; k < meso_patch::MAXF; ++k) {
+            for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
                 if (k >= nface) continue;
                 const int e = it->face[k];
                 const int lf = e & 0xffff;
@@ -610,7 +676,8 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                     F[d] += nei ? -fx : fx;
                 }
             }
-            const CellRec<S> &R = RL ? crl[s] : reinterpret_cast<const CellRec<S> *>(a.crec)[c];
+            const CellRec<S, D> &R = RL ? crl[s] : reinterpret_cast<const CellRec<S, D> *>(a.crec)[c];
+            double qv[3] = {0.0, 0.0, 0.0};
             const double fcg = double(val[s * VT + vl]);
             const double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
             double og, oh = 0.0;
@@ -618,18 +685,17 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 double gsn, hsn, gs, hs;
                 if constexpr (sizeof(S) == 4) {
                     // f_ref flux through the cell: f_ref xi . sum(+-A n), non-zero only for unclosed cells
-                    const float2 cS = RL ? csl[s] : make_float2(a.cell_S[3 * c], a.cell_S[3 * c + 1]);
-                    const double xs = L.x * double(cS.x) + L.y * double(cS.y);
+                    const double xs = cell_xs<D>(RL ? csl[s] : cell_s<D>(a.cell_S + 3 * c), xv);
                     F[0] += L.gref_d * xs;
                     F[1] += L.href_d * xs;
                     float g1, h1, g2, h2;
                     if (TAB) {
                         const float *tb = reinterpret_cast<const float *>(a.cell_tab) + size_t(c) * 2 * a.nt;
-                        shakhov_tab_f(R.en, tb, L.vt, L.xf, 2, a.s.K, L.gref, g1, h1);
-                        shakhov_tab_f(R.e, tb + a.nt, L.vt, L.xf, 2, a.s.K, L.gref, g2, h2);
+                        shakhov_tab_f(R.en, tb, L.vt, L.xf, D, a.s.K, L.gref, g1, h1);
+                        shakhov_tab_f(R.e, tb + a.nt, L.vt, L.xf, D, a.s.K, L.gref, g2, h2);
                     } else {
-                        shakhov_devf(R.en, L.xf, 2, a.s.K, L.gref, g1, h1);
-                        shakhov_devf(R.e, L.xf, 2, a.s.K, L.gref, g2, h2);
+                        shakhov_devf(R.en, L.xf, D, a.s.K, L.gref, g1, h1);
+                        shakhov_devf(R.e, L.xf, D, a.s.K, L.gref, g2, h2);
                     }
                     gsn = g1;
                     hsn = h1;
@@ -643,19 +709,23 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                     og = g + R.Cs * (gs - g);
                     oh = h + R.Cs * (hs - h);
                     const double gf = g + L.gref_d, hf = h + L.href_d;
-                    const double c0 = L.x - R.u0, c1 = L.y - R.u1;
-                    const double tq = 0.5 * L.w * ((c0 * c0 + c1 * c1) * gf + hf);
-                    F[0] = tq * c0;  // reuse as q partials
-                    F[1] = tq * c1;
+                    double cv[D], cc = 0.0;
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        cv[i] = xv[i] - R.u[i];
+                        cc += cv[i] * cv[i];
+                    }
+                    const double tq = 0.5 * L.w * (cc * gf + hf);
+#pragma unroll
+                    for (int i = 0; i < D; ++i) qv[i] = tq * cv[i];
                 } else {
-                    const double x[3] = {L.x, L.y, 0.0};
                     if (TAB) {
                         const double *tb = reinterpret_cast<const double *>(a.cell_tab) + size_t(c) * 2 * a.nt;
-                        shakhov_tab_d(R.en, tb, L.vt, x, 2, a.s.K, gsn, hsn);
-                        shakhov_tab_d(R.e, tb + a.nt, L.vt, x, 2, a.s.K, gs, hs);
+                        shakhov_tab_d(R.en, tb, L.vt, xv, D, a.s.K, gsn, hsn);
+                        shakhov_tab_d(R.e, tb + a.nt, L.vt, xv, D, a.s.K, gs, hs);
                     } else {
-                        shakhov_eq<double>(R.en, x, 2, a.s.K, gsn, hsn);
-                        shakhov_eq<double>(R.e, x, 2, a.s.K, gs, hs);
+                        shakhov_eq<double>(R.en, xv, D, a.s.K, gsn, hsn);
+                        shakhov_eq<double>(R.e, xv, D, a.s.K, gs, hs);
                     }
                     // cdugks_shakhov.cpp:679-695 as written
                     const double gn = R.cm * gsn + (1.0 - R.cm) * fcg;
@@ -664,47 +734,48 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                     const double h = R.C * (hn + hdt * (hs * R.itau + (hsn - hn) * R.itaun) - R.dtv * F[1]);
                     og = (1.0 - R.Cs) * g + R.Cs * gs;
                     oh = (1.0 - R.Cs) * h + R.Cs * hs;
-                    const double c0 = L.x - R.u0, c1 = L.y - R.u1;
-                    const double tq = 0.5 * L.w * ((c0 * c0 + c1 * c1) * g + h);
-                    F[0] = tq * c0;
-                    F[1] = tq * c1;
+                    double cv[D], cc = 0.0;
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        cv[i] = xv[i] - R.u[i];
+                        cc += cv[i] * cv[i];
+                    }
+                    const double tq = 0.5 * L.w * (cc * g + h);
+#pragma unroll
+                    for (int i = 0; i < D; ++i) qv[i] = tq * cv[i];
                 }
             } else {
                 if constexpr (sizeof(S) == 4) {
-                    const float2 cS = RL ? csl[s] : make_float2(a.cell_S[3 * c], a.cell_S[3 * c + 1]);
-                    const double xs = L.x * double(cS.x) + L.y * double(cS.y);
+                    const double xs = cell_xs<D>(RL ? csl[s] : cell_s<D>(a.cell_S + 3 * c), xv);
                     F[0] += L.gref_d * xs;
                     const double feqn = bgk_devf(R.en, L.xf), feq = bgk_devf(R.e, L.xf);
                     const double fn = fcg + R.cm * (feqn - fcg);
                     const double g = fn + R.C * (R.ai * (feq - fn) + R.ai * (feqn - fn) - R.dtv * F[0]);
                     og = g + R.Cs * (feq - g);
                 } else {
-                    const double x[3] = {L.x, L.y, 0.0};
-                    const double feqn = bgk_eq<double>(R.en, x), feq = bgk_eq<double>(R.e, x);
+                    const double feqn = bgk_eq<double>(R.en, xv), feq = bgk_eq<double>(R.e, xv);
                     // cdugks.cpp:325-332 as written
                     const double fn = R.cm * feqn + (1.0 - R.cm) * fcg;
                     og = (1.0 - R.Cs) * (R.C * (fn + hdt * (feq * R.itau + (feqn - fn) * R.itau) - R.dtv * F[0])) +
                          R.Cs * feq;
                 }
-                F[0] = F[1] = 0.0;
             }
             if (L.ok) {
                 fout_g[size_t(c) * nvp + v] = S(og);
                 if (SH) fout_h[size_t(c) * nvp + v] = S(oh);
             } else {
-                F[0] = F[1] = 0.0;
+#pragma unroll
+                for (int i = 0; i < D; ++i) qv[i] = 0.0;
             }
             if (SH) {
                 // sum over the VT lanes of this item
 #pragma unroll
-                for (int o = VT / 2; o > 0; o >>= 1) {
-                    F[0] += __shfl_xor(F[0], o, 64);
-                    F[1] += __shfl_xor(F[1], o, 64);
-                }
-                if (vl == 0) {
-                    qacc[2 * s] += F[0];
-                    qacc[2 * s + 1] += F[1];
-                }
+                for (int o = VT / 2; o > 0; o >>= 1)
+#pragma unroll
+                    for (int i = 0; i < D; ++i) qv[i] += __shfl_xor(qv[i], o, 64);
+                if (vl == 0)
+#pragma unroll
+                    for (int i = 0; i < D; ++i) qacc[D * s + i] += qv[i];
             }
         }
         __syncthreads();
@@ -713,9 +784,6 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
         for (int r = threadIdx.x; r < nown; r += BLOCKP) {
             const int c = items[r].cell;
             double *dst = a.part_c + (size_t(c) * a.s.ntile_c + chunk) * 4;
-            dst[0] = qacc[2 * r];
-            dst[1] = qacc[2 * r + 1];
-            dst[2] = 0.0;
-            dst[3] = 0.0;
+            for (int i = 0; i < 4; ++i) dst[i] = i < D ? qacc[D * r + i] : 0.0;
         }
 }

@@ -1505,8 +1505,9 @@ struct meso_hip_ctx {
     size_t lds_up = 0, lds_mp = 0;
     bool upd_rl = false;      // k_patch_update stages face and cell records in LDS
     bool upd_attr = false;    // its dynamic-LDS limit has been raised
-    meso_patch::Item *up_items = nullptr, *mp_items = nullptr;
-    meso_patch::Face *up_faces = nullptr, *mp_faces = nullptr;
+    bool mp_attr = false;     // the same for k_patch_moments
+    const void *up_items = nullptr, *mp_items = nullptr;  // meso_patch::ItemT<D>
+    const void *up_faces = nullptr, *mp_faces = nullptr;  // meso_patch::FaceT<D>
     int *up_item_ptr = nullptr, *up_face_ptr = nullptr, *up_nown = nullptr, *mp_item_ptr = nullptr,
         *mp_face_ptr = nullptr, *up_row_ptr = nullptr, *up_rows = nullptr, *mp_row_ptr = nullptr,
         *mp_rows = nullptr;
@@ -1828,37 +1829,43 @@ struct Launch {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCKP), shmem, (ctx)->stream, __VA_ARGS__);       \
     } while (0)
 
-// 2D step with the LDS-staged patch kernels for the face moments and the
-// interior cells; boundary faces and boundary cells keep their kernels.
-template <int MODEL, class T, bool TAB>
+// Step with the LDS-staged patch kernels for the face moments and the
+// interior cells; boundary faces and mirror-boundary cells keep their kernels.
+template <int MODEL, class T, bool TAB, int D>
 int step_patch(meso_hip_ctx *c, KArgs a) {
     const StepConst &s = c->s;
     constexpr int WPB = BLOCK / 64;
     constexpr int VTM = sizeof(T) == 4 ? 64 : 32;  // velocities per tile: moments
     constexpr int VTU = sizeof(T) == 4 ? 32 : 16;  // velocities per tile: update
-    constexpr int MAXF = 4;
+    constexpr int MAXF = D == 2 ? 4 : 6;
     const int fwblk = (s.nf + WPB - 1) / WPB;
-    LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM>), c->n_mp * c->p_chunks, c->lds_mp, a);
-    LAUNCH_SH(c, "face_eq", (k_face_eq<MODEL, 2, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
-    if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, 2, T>), s.nb, 0, a);
+    if (!c->mp_attr && c->lds_mp > 65536) {
+        HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_mp)));
+        c->mp_attr = true;
+    }
+    LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D>), c->n_mp * c->p_chunks, c->lds_mp, a);
+    LAUNCH_SH(c, "face_eq", (k_face_eq<MODEL, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
+    if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, D, T>), s.nb, 0, a);
     if (c->n_int > 0) {
-        LAUNCH(c, "cell_prep", (k_cell_prep<MODEL, T, TAB>), (c->n_int + WPB - 1) / WPB, a);
+        LAUNCH(c, "cell_prep", (k_cell_prep<MODEL, T, TAB, D>), (c->n_int + WPB - 1) / WPB, a);
         if (c->upd_rl) {
             if (!c->upd_attr && c->lds_up > 65536) {
-                HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_update<MODEL, T, VTU, TAB, true>),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_up)));
+                HIP_TRY(c, hipFuncSetAttribute(
+                               reinterpret_cast<const void *>(&k_patch_update<MODEL, T, VTU, TAB, true, D>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_up)));
                 c->upd_attr = true;
             }
-            LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, true>), c->n_up * c->p_chunks, c->lds_up,
-                     a);
+            LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, true, D>), c->n_up * c->p_chunks,
+                     c->lds_up, a);
         } else {
-            LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, false>), c->n_up * c->p_chunks, c->lds_up,
-                     a);
+            LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, false, D>), c->n_up * c->p_chunks,
+                     c->lds_up, a);
         }
     }
     const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
     if (c->n_bcell > 0)
-        LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, 2, T, TAB, true>),
+        LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, D, T, TAB, true>),
                   ((c->n_bcell + WPB - 1) / WPB) * s.ntile_c, lds, a, c->n_int, c->n_bcell);
     if (MODEL == CDUGKS_SHAKHOV) LAUNCH(c, "cell_finish", (k_cell_finish<MODEL>), (s.nc + BLOCK - 1) / BLOCK, a);
     HIP_TRY(c, hipGetLastError());
@@ -1871,8 +1878,8 @@ int step_impl(meso_hip_ctx *c, KArgs a) {
     constexpr int WPB = BLOCK / 64;
     constexpr int MF = MODEL == CDUGKS_SHAKHOV ? CDUGKS_SHAKHOV : CDUGKS_BGK;
     constexpr int MAXF = D == 2 ? 4 : 6;
-    if constexpr (D == 2 && MODEL != DUGKS_BGK) {
-        if (c->patches) return step_patch<MODEL, T, TAB>(c, a);
+    if constexpr (MODEL != DUGKS_BGK) {
+        if (c->patches) return step_patch<MODEL, T, TAB, D>(c, a);
     }
     const int fwblk = (s.nf + WPB - 1) / WPB;
     LAUNCH_SH(c, "face_moments", (k_face_moments<MF, D, T>), fwblk * s.ntile_f, 0, a);
@@ -1989,6 +1996,116 @@ int copy_residual_base(meso_hip_ctx *c) {
     HIP_TRY(c, hipMemcpyAsync(c->u_res, c->u[i], 3 * nc * 8, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->T_res, c->T[i], nc * 8, hipMemcpyDeviceToDevice, c->stream));
     HIP_TRY(c, hipMemcpyAsync(c->q_res, c->q[i], 3 * nc * 8, hipMemcpyDeviceToDevice, c->stream));
+    return MESO_OK;
+}
+
+// Spatial patches for the LDS-staged kernels: builds the update and moment
+// patch sets, shrinking the patches until each block fits its LDS budget
+// (half the LDS of a CU, so that two blocks share a CU). Sets the patch fields
+// of c; returns false when the mesh does not fit the patch records.
+template <int D>
+bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<double> &na, int storage,
+                   const std::vector<char> &pown, meso_patch::PatchSet<D> &ups, meso_patch::PatchSet<D> &mps) {
+    const int nc = m->n_cells, nf = m->n_faces, nv = c->s.nv;
+    // velocities per tile (must match step_patch): the moment kernel uses 64
+    // for FP32 storage (one item per wave, records read through the scalar
+    // cache), the update kernel 32 (two items per wave, records in LDS);
+    // FP64 storage halves both
+    const int VTM = storage == MESO_STORAGE_FP32 ? 64 : 32, VTU = VTM / 2;
+    const int IPWM = 64 / VTM, IPWU = 64 / VTU;
+    const size_t es = storage == MESO_STORAGE_FP32 ? 4 : 8;
+    const int ND = c->nd;
+    const int ncol = D == 2 ? (ND == 2 ? 11 : 6) : (ND == 2 ? 17 : 10);
+    constexpr int NWP = BLOCKP / 64;
+    meso_patch::MeshView mv;
+    mv.nc = nc;
+    mv.nf = nf;
+    mv.face_cells = m->face_cells;
+    mv.cell_pos = m->cell_pos;
+    mv.face_pos = m->face_pos;
+    mv.face_area = m->face_area;
+    mv.face_normal = m->face_normal;
+    mv.cf_ptr = m->cell_face_ptr;
+    mv.cf_ids = m->cell_face_ids;
+    mv.nb_ptr = m->nbr_ptr;
+    mv.nb_ids = m->nbr_ids;
+    mv.nb_a = na.data();
+    auto r16 = [](size_t x) { return (x + 15) & ~size_t(15); };
+    std::string perr;
+    const size_t si = sizeof(meso_patch::ItemT<D>), sf = sizeof(meso_patch::FaceT<D>);
+    auto rec = [&](int ipw, int mi, int mf) { return ipw > 1 ? size_t(mi) * si + size_t(mf) * sf : size_t(0); };
+    int lds_blk = 65536, lds_cu = 65536;
+    (void)hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
+    (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device);
+    const size_t cap = std::min<size_t>(size_t(lds_blk), size_t(std::max(lds_cu, lds_blk)) / 2);
+    // The update kernel copies the face equilibria and cell constants of its
+    // patch to LDS as well (RL)
+    c->upd_rl = env_int("MESO_HIP_RECLDS", 1) != 0;
+    bool ok = true;
+    for (int P = env_int("MESO_HIP_PATCH_CELLS", 32); ok; P = P * 3 / 4) {
+        ok = meso_patch::build_update_patches<D>(mv, pown, P, ups, perr);
+        // [items | faces] | rows | values | face fluxes | q partials | [face and cell records] (k_patch_update)
+        c->lds_up = rec(IPWU, ups.max_items, ups.max_faces) + r16(4 * size_t(ups.max_rows)) +
+                    r16(size_t(ND) * ups.max_rows * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
+                    size_t(ups.max_owned) * 8 * D;
+        if (c->upd_rl)
+            c->lds_up += storage == MESO_STORAGE_FP32 ? update_rec_bytes<float, D>(ups.max_faces, ups.max_owned)
+                                                      : update_rec_bytes<double, D>(ups.max_faces, ups.max_owned);
+        if (c->lds_up <= cap || P <= 4) break;
+    }
+    const int fcap = env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
+    for (int F = std::min(fcap, 64); ok; F = F * 3 / 4) {
+        ok = meso_patch::build_moment_patches<D>(mv, F, mps, perr);
+        // [items | faces] | rows | tile velocities | values | f-bar+ rows (k_patch_moments)
+        const size_t a1 = rec(IPWM, mps.max_items, mps.max_faces) + ((4 * size_t(mps.max_rows) + 31) & ~size_t(31)) +
+                          size_t(VTM) * 32 + size_t(ND) * mps.max_rows * VTM * es +
+                          size_t(ND) * mps.max_faces * (VTM + 1) * es;
+        const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer [threads][ncol]
+        c->lds_mp = std::max(a1, a2);
+        if (c->lds_mp <= cap || F <= 4) break;
+    }
+    if (!ok || c->lds_up > cap || c->lds_mp > cap || ups.n_patch == 0 || mps.n_patch == 0) return false;
+    c->patches = true;
+    c->p_vchunk = std::max(64, (env_int("MESO_HIP_PCHUNK", 2560) + 63) / 64 * 64);
+    c->p_chunks = (nv + c->p_vchunk - 1) / c->p_vchunk;
+    c->s.ntile_c = c->s.ntile_f = c->p_chunks;
+    c->s.vt_c = c->s.vt_f = c->p_vchunk;
+    c->n_up = ups.n_patch;
+    c->up_maxi = ups.max_items;
+    c->up_maxf = ups.max_faces;
+    c->up_maxo = ups.max_owned;
+    c->up_maxr = ups.max_rows;
+    c->n_mp = mps.n_patch;
+    c->mp_maxi = mps.max_items;
+    c->mp_maxf = mps.max_faces;
+    c->mp_maxr = mps.max_rows;
+    return true;
+}
+
+template <class P>
+int upload_rec(meso_hip_ctx *c, const void **dst, const std::vector<P> &v) {
+    P *q = nullptr;
+    const int rc = upload(c, &q, v.data(), v.size());
+    *dst = q;
+    return rc;
+}
+
+// Device copies of the patch sets.
+template <int D>
+int upload_patches(meso_hip_ctx *c, const meso_patch::PatchSet<D> &ups, const meso_patch::PatchSet<D> &mps) {
+    int rc;
+    if ((rc = upload_rec(c, &c->up_items, ups.items)) || (rc = upload_rec(c, &c->up_faces, ups.faces)) ||
+        (rc = upload(c, &c->up_item_ptr, ups.item_ptr.data(), ups.item_ptr.size())) ||
+        (rc = upload(c, &c->up_face_ptr, ups.face_ptr.data(), ups.face_ptr.size())) ||
+        (rc = upload(c, &c->up_nown, ups.n_owned.data(), ups.n_owned.size())) ||
+        (rc = upload(c, &c->up_row_ptr, ups.row_ptr.data(), ups.row_ptr.size())) ||
+        (rc = upload(c, &c->up_rows, ups.rows.data(), ups.rows.size())) ||
+        (rc = upload(c, &c->mp_row_ptr, mps.row_ptr.data(), mps.row_ptr.size())) ||
+        (rc = upload(c, &c->mp_rows, mps.rows.data(), mps.rows.size())) ||
+        (rc = upload_rec(c, &c->mp_items, mps.items)) || (rc = upload_rec(c, &c->mp_faces, mps.faces)) ||
+        (rc = upload(c, &c->mp_item_ptr, mps.item_ptr.data(), mps.item_ptr.size())) ||
+        (rc = upload(c, &c->mp_face_ptr, mps.face_ptr.data(), mps.face_ptr.size())))
+        return rc;
     return MESO_OK;
 }
 
@@ -2333,37 +2450,10 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
                 for (int k = 0; k < 3; ++k) mirror_s[3 * size_t(sl) + k] = slot_of[d->mirror[3 * size_t(perm[sl]) + k]];
         }
     }
-    // ---- LDS-staged patch kernels: 2D CDUGKS models with large velocity sets
-    meso_patch::PatchSet ups, mps;
-    if (D == 2 && p->model != MESO_MODEL_DUGKS_BGK && nv >= 64 && env_int("MESO_HIP_PATCH", 1) != 0) {
-        // velocities per tile (must match step_patch): the moment kernel uses
-        // 64 for FP32 storage (one item per wave, records read through the
-        // scalar cache), the update kernel 32 (two items per wave, records in
-        // LDS); FP64 storage halves both
-        const int VTM = storage == MESO_STORAGE_FP32 ? 64 : 32, VTU = VTM / 2;
-        const int IPWM = 64 / VTM, IPWU = 64 / VTU;
-        const size_t es = storage == MESO_STORAGE_FP32 ? 4 : 8;
-        const int ND = c->nd;
-        const int ncol = c->nd == 2 ? 11 : 6;
-        constexpr int NWP = BLOCKP / 64;
-        const size_t cap = 65536;
-        meso_patch::MeshView mv;
-        mv.nc = nc;
-        mv.nf = nf;
-        mv.face_cells = m->face_cells;
-        mv.cell_pos = m->cell_pos;
-        mv.face_pos = m->face_pos;
-        mv.face_area = m->face_area;
-        mv.face_normal = m->face_normal;
-        mv.cf_ptr = m->cell_face_ptr;
-        mv.cf_ids = m->cell_face_ids;
-        mv.nb_ptr = m->nbr_ptr;
-        mv.nb_ids = m->nbr_ids;
-        mv.nb_a = na.data();
-        auto r16 = [](size_t x) { return (x + 15) & ~size_t(15); };
-        std::string perr;
-        bool ok = true;
-        const size_t si = sizeof(meso_patch::Item), sf = sizeof(meso_patch::Face);
+    // ---- LDS-staged patch kernels: CDUGKS models with large velocity sets
+    meso_patch::PatchSet<2> ups2, mps2;
+    meso_patch::PatchSet<3> ups3, mps3;
+    if (p->model != MESO_MODEL_DUGKS_BGK && nv >= 64 && env_int("MESO_HIP_PATCH", 1) != 0) {
         // every cell is updated by a patch except those with a mirror boundary
         // (symmetry, slip-wall), whose face values need other velocities
         std::vector<char> pown(nc, 1);
@@ -2373,52 +2463,9 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             const int kind = m->marks[m->face_mark[f]].type;
             if (kind == MESO_BC_SYMMETRY || kind == MESO_BC_SLIP_WALL) pown[m->face_cells[2 * f]] = 0;
         }
-        auto rec = [&](int ipw, int mi, int mf) { return ipw > 1 ? size_t(mi) * si + size_t(mf) * sf : size_t(0); };
-        // The update kernel copies the face equilibria and cell constants of its
-        // patch to LDS as well; it may take up to half the LDS of a CU, so that
-        // two blocks share a CU
-        int lds_blk = 65536, lds_cu = 65536;
-        (void)hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
-        (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device);
-        c->upd_rl = env_int("MESO_HIP_RECLDS", 1) != 0;
-        const size_t cap_up = c->upd_rl ? std::min<size_t>(size_t(lds_blk), size_t(std::max(lds_cu, lds_blk)) / 2) : cap;
-        for (int P = env_int("MESO_HIP_PATCH_CELLS", 32); ok; P = P * 3 / 4) {
-            ok = meso_patch::build_update_patches(mv, pown, P, ups, perr);
-            // [items | faces] | rows | values | face fluxes | q partials | [face and cell records] (k_patch_update)
-            c->lds_up = rec(IPWU, ups.max_items, ups.max_faces) + r16(4 * size_t(ups.max_rows)) +
-                        r16(size_t(ND) * ups.max_rows * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
-                        size_t(ups.max_owned) * 16;
-            if (c->upd_rl)
-                c->lds_up += storage == MESO_STORAGE_FP32 ? update_rec_bytes<float>(ups.max_faces, ups.max_owned)
-                                                          : update_rec_bytes<double>(ups.max_faces, ups.max_owned);
-            if (c->lds_up <= cap_up || P <= 4) break;
-        }
-        const int fcap = env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
-        for (int F = std::min(fcap, 64); ok; F = F * 3 / 4) {
-            ok = meso_patch::build_moment_patches(mv, F, mps, perr);
-            // [items | faces] | rows | tile velocities | values | f-bar+ rows (k_patch_moments)
-            const size_t a1 = rec(IPWM, mps.max_items, mps.max_faces) +
-                              ((4 * size_t(mps.max_rows) + 31) & ~size_t(31)) + size_t(VTM) * 32 +
-                              size_t(ND) * mps.max_rows * VTM * es + size_t(ND) * mps.max_faces * (VTM + 1) * es;
-            const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer [threads][ncol]
-            c->lds_mp = std::max(a1, a2);
-            if (c->lds_mp <= cap || F <= 4) break;
-        }
-        if (ok && c->lds_up <= cap_up && c->lds_mp <= cap && ups.n_patch > 0 && mps.n_patch > 0) {
-            c->patches = true;
-            c->p_vchunk = std::max(64, (env_int("MESO_HIP_PCHUNK", 2560) + 63) / 64 * 64);
-            c->p_chunks = (nv + c->p_vchunk - 1) / c->p_vchunk;
-            c->s.ntile_c = c->s.ntile_f = c->p_chunks;
-            c->s.vt_c = c->s.vt_f = c->p_vchunk;
-            c->n_up = ups.n_patch;
-            c->up_maxi = ups.max_items;
-            c->up_maxf = ups.max_faces;
-            c->up_maxo = ups.max_owned;
-            c->up_maxr = ups.max_rows;
-            c->n_mp = mps.n_patch;
-            c->mp_maxi = mps.max_items;
-            c->mp_maxf = mps.max_faces;
-            c->mp_maxr = mps.max_rows;
+        const bool ok = D == 2 ? setup_patches<2>(c, m, na, storage, pown, ups2, mps2)
+                               : setup_patches<3>(c, m, na, storage, pown, ups3, mps3);
+        if (ok) {
             // cell lists: patch cells first (k_cell_prep), then the mirror-boundary cells (k_cell_update)
             t_list.clear();
             for (int cc = 0; cc < nc; ++cc)
@@ -2471,19 +2518,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     UP(fs_a, t_fsa.data(), t_fsa.size());
     UP(fs_af, t_fsaf.data(), t_fsaf.size());
     if (c->patches) {
-        UP(up_items, ups.items.data(), ups.items.size());
-        UP(up_faces, ups.faces.data(), ups.faces.size());
-        UP(up_item_ptr, ups.item_ptr.data(), ups.item_ptr.size());
-        UP(up_face_ptr, ups.face_ptr.data(), ups.face_ptr.size());
-        UP(up_nown, ups.n_owned.data(), ups.n_owned.size());
-        UP(up_row_ptr, ups.row_ptr.data(), ups.row_ptr.size());
-        UP(up_rows, ups.rows.data(), ups.rows.size());
-        UP(mp_row_ptr, mps.row_ptr.data(), mps.row_ptr.size());
-        UP(mp_rows, mps.rows.data(), mps.rows.size());
-        UP(mp_items, mps.items.data(), mps.items.size());
-        UP(mp_faces, mps.faces.data(), mps.faces.size());
-        UP(mp_item_ptr, mps.item_ptr.data(), mps.item_ptr.size());
-        UP(mp_face_ptr, mps.face_ptr.data(), mps.face_ptr.size());
+        if ((rc = D == 2 ? upload_patches<2>(c, ups2, mps2) : upload_patches<3>(c, ups3, mps3))) return fail(rc);
     }
     UP(vel_d, t_vd.data(), t_vd.size());
     UP(vel_f, t_vf.data(), t_vf.size());
@@ -2535,7 +2570,9 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     }
     if (c->patches) {
         char *cr = nullptr;
-        const size_t rs = storage == MESO_STORAGE_FP32 ? sizeof(CellRec<float>) : sizeof(CellRec<double>);
+        const size_t rs = storage == MESO_STORAGE_FP32
+                              ? (D == 2 ? sizeof(CellRec<float, 2>) : sizeof(CellRec<float, 3>))
+                              : (D == 2 ? sizeof(CellRec<double, 2>) : sizeof(CellRec<double, 3>));
         if ((rc = dalloc(c, &cr, size_t(nc) * rs))) return fail(rc);
         c->crec = cr;
         if (c->nt > 0) {

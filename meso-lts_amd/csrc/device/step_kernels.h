@@ -114,12 +114,12 @@ struct KArgs {
     // moment patches (all faces); the velocity range is split into p_chunks
     // chunks of p_vchunk velocities
     int n_up, up_maxi, up_maxf, up_maxo, up_maxr;
-    const meso_patch::Item *up_items;
-    const meso_patch::Face *up_faces;
+    const void *up_items;         // meso_patch::ItemT<D>
+    const void *up_faces;         // meso_patch::FaceT<D>
     const int *up_item_ptr, *up_face_ptr, *up_nown, *up_row_ptr, *up_rows;
     int n_mp, mp_maxi, mp_maxf, mp_maxr;
-    const meso_patch::Item *mp_items;
-    const meso_patch::Face *mp_faces;
+    const void *mp_items;
+    const void *mp_faces;
     const int *mp_item_ptr, *mp_face_ptr, *mp_row_ptr, *mp_rows;
     int p_chunks, p_vchunk;     // velocity chunks of the patch kernels and their size (a multiple of 64)
     int n_int_cells;            // interior cells: cell_list[0, n_int_cells)

@@ -1,4 +1,4 @@
-// Spatial patches of a 2D mesh for the LDS-staged step kernels (see patches.h).
+// Spatial patches of a 2D or 3D mesh for the LDS-staged step kernels (see patches.h).
 #include "patches.h"
 
 #include <algorithm>
@@ -9,7 +9,7 @@ namespace meso_patch {
 
 namespace {
 
-void rcb_rec(std::vector<int> &ids, size_t lo, size_t hi, const double *pos, int stride, int cap,
+void rcb_rec(std::vector<int> &ids, size_t lo, size_t hi, const double *pos, int stride, int cap, int dims,
              std::vector<std::vector<int>> &out) {
     const size_t n = hi - lo;
     if (n <= size_t(cap)) {
@@ -19,32 +19,38 @@ void rcb_rec(std::vector<int> &ids, size_t lo, size_t hi, const double *pos, int
     const size_t k = (n + cap - 1) / cap;  // groups this range becomes
     const size_t k1 = k / 2;
     const size_t n1 = (n * k1 + k / 2) / k;  // cells of the first k1 groups, proportional
-    double mn[2] = {1e300, 1e300}, mx[2] = {-1e300, -1e300};
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
     for (size_t i = lo; i < hi; ++i)
-        for (int a = 0; a < 2; ++a) {
+        for (int a = 0; a < dims; ++a) {
             const double x = pos[size_t(ids[i]) * stride + a];
             mn[a] = std::min(mn[a], x);
             mx[a] = std::max(mx[a], x);
         }
-    const int ax = (mx[0] - mn[0] >= mx[1] - mn[1]) ? 0 : 1;
+    int ax = 0;
+    for (int a = 1; a < dims; ++a)
+        if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
     auto key = [&](int id) { return pos[size_t(id) * stride + ax]; };
     std::nth_element(ids.begin() + lo, ids.begin() + lo + n1, ids.begin() + hi, [&](int a, int b) {
         const double ka = key(a), kb = key(b);
         return ka < kb || (ka == kb && a < b);
     });
-    rcb_rec(ids, lo, lo + n1, pos, stride, cap, out);
-    rcb_rec(ids, lo + n1, hi, pos, stride, cap, out);
+    rcb_rec(ids, lo, lo + n1, pos, stride, cap, dims, out);
+    rcb_rec(ids, lo + n1, hi, pos, stride, cap, dims, out);
 }
 
+template <int D>
 struct Builder {
+    using Item = ItemT<D>;
+    using Face = FaceT<D>;
+    static constexpr int MAXN = Lim<D>::MAXN;
     const MeshView &m;
-    PatchSet &ps;
+    PatchSet<D> &ps;
     std::unordered_map<int, int> loc;  // global cell -> item index in the current patch
     std::vector<Item> items;
     std::vector<Face> faces;
     std::unordered_map<int, int> floc;  // global face -> local face index
 
-    Builder(const MeshView &m_, PatchSet &p) : m(m_), ps(p) {}
+    Builder(const MeshView &m_, PatchSet<D> &p) : m(m_), ps(p) {}
 
     int add_item(int c) {
         auto it = loc.find(c);
@@ -93,10 +99,10 @@ struct Builder {
                         rows.push_back(g);
                     }
                     it.nb[j] = f->second;
-                    for (int a = 0; a < 2; ++a) it.a[j][a] = m.nb_a[3 * size_t(j0 + j) + a];
+                    for (int a = 0; a < D; ++a) it.a[j][a] = m.nb_a[3 * size_t(j0 + j) + a];
                 } else {
                     it.nb[j] = self;
-                    it.a[j][0] = it.a[j][1] = 0.0;
+                    for (int a = 0; a < D; ++a) it.a[j][a] = 0.0;
                 }
             }
         }
@@ -106,9 +112,8 @@ struct Builder {
             fr.own = loc.at(o);
             fr.nei = loc.at(n);
             fr.area = m.face_area[f];
-            fr.n[0] = m.face_normal[3 * size_t(f)];
-            fr.n[1] = m.face_normal[3 * size_t(f) + 1];
-            for (int a = 0; a < 2; ++a) {
+            for (int a = 0; a < D; ++a) fr.n[a] = m.face_normal[3 * size_t(f) + a];
+            for (int a = 0; a < D; ++a) {
                 fr.d[0][a] = m.face_pos[3 * size_t(f) + a] - m.cell_pos[3 * size_t(o) + a];
                 fr.d[1][a] = m.face_pos[3 * size_t(f) + a] - m.cell_pos[3 * size_t(n) + a];
             }
@@ -132,18 +137,20 @@ struct Builder {
     }
 };
 
+template <int D>
 bool check_limits(const MeshView &m, std::string &err) {
     for (int c = 0; c < m.nc; ++c) {
-        if (m.cf_ptr[c + 1] - m.cf_ptr[c] > MAXF || m.nb_ptr[c + 1] - m.nb_ptr[c] > MAXN) {
-            err = "patch kernels: a cell has more than 4 faces or LSQ neighbours";
+        if (m.cf_ptr[c + 1] - m.cf_ptr[c] > Lim<D>::MAXF || m.nb_ptr[c + 1] - m.nb_ptr[c] > Lim<D>::MAXN) {
+            err = "patch kernels: a cell has more faces or LSQ neighbours than the patch records hold";
             return false;
         }
     }
     return true;
 }
 
-void reset(PatchSet &ps) {
-    ps = PatchSet();
+template <int D>
+void reset(PatchSet<D> &ps) {
+    ps = PatchSet<D>();
     ps.item_ptr.push_back(0);
     ps.face_ptr.push_back(0);
     ps.row_ptr.push_back(0);
@@ -151,23 +158,24 @@ void reset(PatchSet &ps) {
 
 }  // namespace
 
-std::vector<std::vector<int>> rcb(const std::vector<int> &ids, const double *pos, int stride, int cap) {
+std::vector<std::vector<int>> rcb(const std::vector<int> &ids, const double *pos, int stride, int cap, int dims) {
     std::vector<std::vector<int>> out;
     if (ids.empty()) return out;
     std::vector<int> work(ids);
-    rcb_rec(work, 0, work.size(), pos, stride, std::max(cap, 1), out);
+    rcb_rec(work, 0, work.size(), pos, stride, std::max(cap, 1), dims, out);
     return out;
 }
 
-bool build_update_patches(const MeshView &m, const std::vector<char> &own, int max_owned, PatchSet &ps,
+template <int D>
+bool build_update_patches(const MeshView &m, const std::vector<char> &own, int max_owned, PatchSet<D> &ps,
                           std::string &err) {
     reset(ps);
-    if (!check_limits(m, err)) return false;
+    if (!check_limits<D>(m, err)) return false;
     std::vector<int> owned;
     for (int c = 0; c < m.nc; ++c)
         if (own[c]) owned.push_back(c);
-    Builder b(m, ps);
-    for (const auto &grp : rcb(owned, m.cell_pos, 3, max_owned)) {
+    Builder<D> b(m, ps);
+    for (const auto &grp : rcb(owned, m.cell_pos, 3, max_owned, D)) {
         for (int c : grp) b.add_item(c);
         // faces of the owned cells in meso's cell-face order, and the cells across them
         for (int c : grp)
@@ -178,7 +186,7 @@ bool build_update_patches(const MeshView &m, const std::vector<char> &own, int m
                 b.add_item(o == c ? n : o);
             }
         for (size_t i = 0; i < b.items.size(); ++i) {
-            Item &it = b.items[i];
+            auto &it = b.items[i];
             const int c = it.cell;
             for (int k = m.cf_ptr[c]; k < m.cf_ptr[c + 1]; ++k) {
                 const int f = m.cf_ids[k];
@@ -190,20 +198,21 @@ bool build_update_patches(const MeshView &m, const std::vector<char> &own, int m
     return true;
 }
 
-bool build_moment_patches(const MeshView &m, int max_faces, PatchSet &ps, std::string &err) {
+template <int D>
+bool build_moment_patches(const MeshView &m, int max_faces, PatchSet<D> &ps, std::string &err) {
     reset(ps);
-    if (!check_limits(m, err)) return false;
+    if (!check_limits<D>(m, err)) return false;
     std::vector<int> all(m.nf);
     for (int f = 0; f < m.nf; ++f) all[f] = f;
-    Builder b(m, ps);
-    for (const auto &grp : rcb(all, m.face_pos, 3, max_faces)) {
+    Builder<D> b(m, ps);
+    for (const auto &grp : rcb(all, m.face_pos, 3, max_faces, D)) {
         for (int f : grp) {
             b.add_face(f);
             b.add_item(m.face_cells[2 * size_t(f)]);
             b.add_item(m.face_cells[2 * size_t(f) + 1]);
         }
         for (size_t i = 0; i < b.items.size(); ++i) {
-            Item &it = b.items[i];
+            auto &it = b.items[i];
             const int c = it.cell;
             for (int k = m.cf_ptr[c]; k < m.cf_ptr[c + 1]; ++k) {
                 const int f = m.cf_ids[k];
@@ -214,5 +223,10 @@ bool build_moment_patches(const MeshView &m, int max_faces, PatchSet &ps, std::s
     }
     return true;
 }
+
+template bool build_update_patches<2>(const MeshView &, const std::vector<char> &, int, PatchSet<2> &, std::string &);
+template bool build_update_patches<3>(const MeshView &, const std::vector<char> &, int, PatchSet<3> &, std::string &);
+template bool build_moment_patches<2>(const MeshView &, int, PatchSet<2> &, std::string &);
+template bool build_moment_patches<3>(const MeshView &, int, PatchSet<3> &, std::string &);
 
 }  // namespace meso_patch
