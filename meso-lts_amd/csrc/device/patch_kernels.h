@@ -196,40 +196,6 @@ __device__ __forceinline__ LaneVel lane_vel(const KArgs &a, int v) {
     return L;
 }
 
-// Equilibrium pair of face f at the lane velocity, in the storage form
-// (deviation for FP32, full value for FP64), as FP64.
-template <int MODEL, class S, bool TAB, int D>
-__device__ __forceinline__ void face_eq_lane(const KArgs &a, int f, const LaneVel &L, double &gs, double &hs) {
-    if constexpr (sizeof(S) == 4) {
-        const EqF &e = a.face_eqf[f];
-        float g, h = 0.0f;
-        if (MODEL == CDUGKS_SHAKHOV) {
-            if (TAB)
-                shakhov_tab_f(e, reinterpret_cast<const float *>(a.face_tab) + size_t(f) * a.nt, L.vt, L.xf, D, a.s.K,
-                              L.gref, g, h);
-            else
-                shakhov_devf(e, L.xf, D, a.s.K, L.gref, g, h);
-        } else {
-            g = bgk_devf(e, L.xf);
-        }
-        gs = g;
-        hs = h;
-    } else {
-        const EqParams &e = a.face_eq[f];
-        const double x[3] = {L.x, L.y, L.z};
-        if (MODEL == CDUGKS_SHAKHOV) {
-            if (TAB)
-                shakhov_tab_d(e, reinterpret_cast<const double *>(a.face_tab) + size_t(f) * a.nt, L.vt, x, D, a.s.K,
-                              gs, hs);
-            else
-                shakhov_eq<double>(e, x, D, a.s.K, gs, hs);
-        } else {
-            gs = bgk_eq<double>(e, x);
-            hs = 0.0;
-        }
-    }
-}
-
 // Face equilibrium record of the storage kind (FP32 copy or FP64).
 template <class S>
 struct UpdateRecs;
@@ -271,14 +237,15 @@ __device__ __forceinline__ double cell_xs(const CellS<D> &s, const double *xv) {
     return r;
 }
 
-// Bytes of LDS the face and cell records of k_patch_update take with RL.
+// Bytes of LDS the face and cell records of k_patch_update take.
 template <class S, int D>
 __host__ __device__ constexpr size_t update_rec_bytes(int maxf, int maxo) {
     return size_t(maxf) * (sizeof(typename UpdateRecs<S>::Eq) + 8) +
            size_t(maxo) * (sizeof(CellRec<S, D>) + sizeof(CellS<D>));
 }
 
-// face_eq_lane with the face's equilibrium record e at hand.
+// Equilibrium pair of face f at the lane velocity from its record e, in the
+// storage form (deviation for FP32, full value for FP64), as FP64.
 template <int MODEL, class S, bool TAB, int D>
 __device__ __forceinline__ void face_eq_rec(const KArgs &a, const typename UpdateRecs<S>::Eq &e, int f,
                                             const LaneVel &L, double &gs, double &hs) {
@@ -521,11 +488,11 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
 
 // ------------------------------------------------------------- update
 // Cells of the update patches: cell flux, f-tilde-plus update and the Shakhov
-// heat-flux partials part_c[cell][chunk][0..1]. RL: the equilibrium records of
+// heat-flux partials part_c[cell][chunk][0..D-1]. The equilibrium records of
 // the patch faces and the update constants of the owned cells are copied to
-// LDS once per block; otherwise every tile re-reads them per lane from HBM
-// (L2), which costs address arithmetic and L2 latency in every tile.
-template <int MODEL, class S, int VT, bool TAB, bool RL, int D>
+// LDS once per block (read per lane from HBM in every tile they would cost
+// address arithmetic and L2 latency).
+template <int MODEL, class S, int VT, bool TAB, int D>
 __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.halt) return;
@@ -554,8 +521,8 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     S *flx = val + size_t(ND) * maxr * VT;
     double *qacc = reinterpret_cast<double *>(
         lds_raw + ((reinterpret_cast<unsigned char *>(flx + size_t(ND) * maxf * VT) - lds_raw + 15) & ~15));
-    // with RL: the face equilibria and mix weights of the patch faces and the
-    // update constants of the owned cells, copied once per block
+    // the face equilibria and mix weights of the patch faces and the update
+    // constants of the owned cells, copied once per block
     using EqS = typename UpdateRecs<S>::Eq;
     EqS *feq = reinterpret_cast<EqS *>(qacc + D * a.up_maxo);
     double *fmix = reinterpret_cast<double *>(feq + maxf);
@@ -572,17 +539,15 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     stage_copy(up_faces + f0, nfp, faces);
     for (int r = threadIdx.x; r < nr; r += BLOCKP) rows[r] = a.up_rows[r0 + r];
     for (int r = threadIdx.x; r < D * nown; r += BLOCKP) qacc[r] = 0.0;
-    if constexpr (RL) {
-        for (int lf = threadIdx.x; lf < nfp; lf += BLOCKP) {
-            const int f = up_faces[f0 + lf].face;
-            feq[lf] = UpdateRecs<S>::face_eq(a, f);
-            fmix[lf] = a.face_eq[f].mix;
-        }
-        for (int r = threadIdx.x; r < nown; r += BLOCKP) {
-            const int c = up_items[i0 + r].cell;
-            crl[r] = reinterpret_cast<const CellRec<S, D> *>(a.crec)[c];
-            csl[r] = cell_s<D>(a.cell_S + 3 * c);
-        }
+    for (int lf = threadIdx.x; lf < nfp; lf += BLOCKP) {
+        const int f = up_faces[f0 + lf].face;
+        feq[lf] = UpdateRecs<S>::face_eq(a, f);
+        fmix[lf] = a.face_eq[f].mix;
+    }
+    for (int r = threadIdx.x; r < nown; r += BLOCKP) {
+        const int c = up_items[i0 + r].cell;
+        crl[r] = reinterpret_cast<const CellRec<S, D> *>(a.crec)[c];
+        csl[r] = cell_s<D>(a.cell_S + 3 * c);
     }
     const int ntiles = (a.s.nv + VT - 1) / VT;
     const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
@@ -623,14 +588,8 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 if (up) {
                     const int f = F->face;
                     double gs, hs = 0.0;
-                    double mix;
-                    if constexpr (RL) {
-                        face_eq_rec<MODEL, S, TAB, D>(a, feq[lf], f, L, gs, hs);
-                        mix = fmix[lf];
-                    } else {
-                        face_eq_lane<MODEL, S, TAB, D>(a, f, L, gs, hs);
-                        mix = a.face_eq[f].mix;
-                    }
+                    face_eq_rec<MODEL, S, TAB, D>(a, feq[lf], f, L, gs, hs);
+                    const double mix = fmix[lf];
                     const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
                     double ev[D];
 #pragma unroll
@@ -676,7 +635,7 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                     F[d] += nei ? -fx : fx;
                 }
             }
-            const CellRec<S, D> &R = RL ? crl[s] : reinterpret_cast<const CellRec<S, D> *>(a.crec)[c];
+            const CellRec<S, D> &R = crl[s];
             double qv[3] = {0.0, 0.0, 0.0};
             const double fcg = double(val[s * VT + vl]);
             const double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
@@ -685,7 +644,7 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 double gsn, hsn, gs, hs;
                 if constexpr (sizeof(S) == 4) {
                     // f_ref flux through the cell: f_ref xi . sum(+-A n), non-zero only for unclosed cells
-                    const double xs = cell_xs<D>(RL ? csl[s] : cell_s<D>(a.cell_S + 3 * c), xv);
+                    const double xs = cell_xs<D>(csl[s], xv);
                     F[0] += L.gref_d * xs;
                     F[1] += L.href_d * xs;
                     float g1, h1, g2, h2;
@@ -746,7 +705,7 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 }
             } else {
                 if constexpr (sizeof(S) == 4) {
-                    const double xs = cell_xs<D>(RL ? csl[s] : cell_s<D>(a.cell_S + 3 * c), xv);
+                    const double xs = cell_xs<D>(csl[s], xv);
                     F[0] += L.gref_d * xs;
                     const double feqn = bgk_devf(R.en, L.xf), feq = bgk_devf(R.e, L.xf);
                     const double fn = fcg + R.cm * (feqn - fcg);

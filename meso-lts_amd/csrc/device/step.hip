@@ -1503,8 +1503,7 @@ struct meso_hip_ctx {
         mp_maxr = 0;
     int p_chunks = 0, p_vchunk = 0;
     size_t lds_up = 0, lds_mp = 0;
-    bool upd_rl = false;      // k_patch_update stages face and cell records in LDS
-    bool upd_attr = false;    // its dynamic-LDS limit has been raised
+    bool upd_attr = false;    // the dynamic-LDS limit of k_patch_update has been raised
     bool mp_attr = false;     // the same for k_patch_moments
     const void *up_items = nullptr, *mp_items = nullptr;  // meso_patch::ItemT<D>
     const void *up_faces = nullptr, *mp_faces = nullptr;  // meso_patch::FaceT<D>
@@ -1849,19 +1848,12 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
     if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, D, T>), s.nb, 0, a);
     if (c->n_int > 0) {
         LAUNCH(c, "cell_prep", (k_cell_prep<MODEL, T, TAB, D>), (c->n_int + WPB - 1) / WPB, a);
-        if (c->upd_rl) {
-            if (!c->upd_attr && c->lds_up > 65536) {
-                HIP_TRY(c, hipFuncSetAttribute(
-                               reinterpret_cast<const void *>(&k_patch_update<MODEL, T, VTU, TAB, true, D>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_up)));
-                c->upd_attr = true;
-            }
-            LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, true, D>), c->n_up * c->p_chunks,
-                     c->lds_up, a);
-        } else {
-            LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, false, D>), c->n_up * c->p_chunks,
-                     c->lds_up, a);
+        if (!c->upd_attr && c->lds_up > 65536) {
+            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_update<MODEL, T, VTU, TAB, D>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_up)));
+            c->upd_attr = true;
         }
+        LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, D>), c->n_up * c->p_chunks, c->lds_up, a);
     }
     const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
     if (c->n_bcell > 0)
@@ -2038,20 +2030,21 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     (void)hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device);
     const size_t cap = std::min<size_t>(size_t(lds_blk), size_t(std::max(lds_cu, lds_blk)) / 2);
-    // The update kernel copies the face equilibria and cell constants of its
-    // patch to LDS as well (RL)
-    c->upd_rl = env_int("MESO_HIP_RECLDS", 1) != 0;
+    // 3D FP64 update blocks take the whole LDS of a CU (one block per CU): the
+    // 16-velocity tiles then serve 32-cell patches instead of 9-cell ones,
+    // with 2.8 instead of 3.8 items per owned cell (18.4 -> 13.8 ms for 32^3
+    // cells x 16^3 velocities); FP32 (32-velocity tiles) gains nothing from it
+    const size_t cap_up = D == 3 && storage == MESO_STORAGE_FP64 ? size_t(lds_blk) : cap;
     bool ok = true;
     for (int P = env_int("MESO_HIP_PATCH_CELLS", 32); ok; P = P * 3 / 4) {
         ok = meso_patch::build_update_patches<D>(mv, pown, P, ups, perr);
-        // [items | faces] | rows | values | face fluxes | q partials | [face and cell records] (k_patch_update)
+        // items | faces | rows | values | face fluxes | q partials | face and cell records (k_patch_update)
         c->lds_up = rec(IPWU, ups.max_items, ups.max_faces) + r16(4 * size_t(ups.max_rows)) +
                     r16(size_t(ND) * ups.max_rows * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
-                    size_t(ups.max_owned) * 8 * D;
-        if (c->upd_rl)
-            c->lds_up += storage == MESO_STORAGE_FP32 ? update_rec_bytes<float, D>(ups.max_faces, ups.max_owned)
-                                                      : update_rec_bytes<double, D>(ups.max_faces, ups.max_owned);
-        if (c->lds_up <= cap || P <= 4) break;
+                    size_t(ups.max_owned) * 8 * D +
+                    (storage == MESO_STORAGE_FP32 ? update_rec_bytes<float, D>(ups.max_faces, ups.max_owned)
+                                                  : update_rec_bytes<double, D>(ups.max_faces, ups.max_owned));
+        if (c->lds_up <= cap_up || P <= 4) break;
     }
     const int fcap = env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
     for (int F = std::min(fcap, 64); ok; F = F * 3 / 4) {
@@ -2064,7 +2057,7 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
         c->lds_mp = std::max(a1, a2);
         if (c->lds_mp <= cap || F <= 4) break;
     }
-    if (!ok || c->lds_up > cap || c->lds_mp > cap || ups.n_patch == 0 || mps.n_patch == 0) return false;
+    if (!ok || c->lds_up > cap_up || c->lds_mp > cap || ups.n_patch == 0 || mps.n_patch == 0) return false;
     c->patches = true;
     c->p_vchunk = std::max(64, (env_int("MESO_HIP_PCHUNK", 2560) + 63) / 64 * 64);
     c->p_chunks = (nv + c->p_vchunk - 1) / c->p_vchunk;
