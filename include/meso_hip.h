@@ -129,6 +129,14 @@ typedef struct meso_step_params {
     double Pr, R, T0, mu0, omega;
     /* incompressible BGK */
     double RT, tau;
+    /* Shakhov: Venkatakrishnan limiter of the reconstruction
+     * (src/solver/limiter.cpp:7-31, applied at cdugks_shakhov.cpp:231-266).
+     * venkata_k is meso's venkata_k member: read from `venkata-limiter-k` only
+     * when the solver is built with gradient-switch False and limiter-switch
+     * True (cdugks_shakhov.cpp:21-28), 0 otherwise. */
+    int32_t limiter_switch;
+    int32_t reserved;
+    double venkata_k;
 } meso_step_params;
 
 /* Result of meso_hip_step(). */

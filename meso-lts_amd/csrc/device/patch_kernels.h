@@ -112,6 +112,20 @@ __device__ __forceinline__ void item_grad(const meso_patch::ItemT<D> *it, const 
     }
 }
 
+// Value range [mn, mx] of one item and its LSQ (face) neighbours from the
+// staged rows (padding slots repeat the item); for the limiter.
+template <class S, int VT, int D>
+__device__ __forceinline__ void item_range(const meso_patch::ItemT<D> *it, const S *val_d, int vl, double fc,
+                                           double &mn, double &mx) {
+    mn = mx = fc;
+#pragma unroll
+    for (int j = 0; j < meso_patch::Lim<D>::MAXN; ++j) {
+        const double w = double(val_d[it->nb[j] * VT + vl]);
+        mn = fmin(mn, w);
+        mx = fmax(mx, w);
+    }
+}
+
 // Block prologue: copy n records of R bytes (a multiple of 16) to LDS.
 template <class R>
 __device__ __forceinline__ void stage_copy(const R *src, int n, R *dst) {
@@ -391,14 +405,16 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         for (int s = slot; s < ni; s += NSLOT) {
             const Item *it = rec.item(s);
             const int nface = it->nface;
-            double fc[2], g[2][D];
+            double fc[2], g[2][D], mn[2], mx[2];
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 fc[d] = double(val[(d * maxr + s) * VT + vl]);
 #pragma unroll
                 for (int k = 0; k < D; ++k) g[d][k] = 0.0;
                 if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d]);
+                if (a.s.limiter) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d]);
             }
+            const double omega = a.s.limiter ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
             const double xv[3] = {X.x, X.y, X.z};
 #pragma unroll
             for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
@@ -422,6 +438,12 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                     double Xd = 0.0;
 #pragma unroll
                     for (int i = 0; i < D; ++i) Xd += ev[i] * g[d][i];
+                    if (a.s.limiter) {
+                        double dw = 0.0;
+#pragma unroll
+                        for (int i = 0; i < D; ++i) dw += F->d[side][i] * g[d][i];
+                        Xd *= venkata_phi(dw, fc[d], mn[d], mx[d], omega, sn >= VSMALL ? 1.0e-10 : VSMALL);
+                    }
                     S *dst = &fb[(d * maxf + lf) * VTP + vl];
                     if (mode == meso_patch::SIDE_BOUNDARY) {
                         *dst = S(dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd) : fc[d] + Xd);
@@ -563,14 +585,16 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
         for (int s = slot; s < ni; s += NSLOT) {
             const Item *it = items + s;
             const int nface = it->nface;
-            double fc[2], g[2][D];
+            double fc[2], g[2][D], mn[2], mx[2];
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 fc[d] = double(val[(d * maxr + s) * VT + vl]);
 #pragma unroll
                 for (int k = 0; k < D; ++k) g[d][k] = 0.0;
                 if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d]);
+                if (a.s.limiter) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d]);
             }
+            const double omega = a.s.limiter ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
 #pragma unroll
             for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
                 if (k >= nface) continue;
@@ -601,6 +625,12 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                         double Xd = 0.0;
 #pragma unroll
                         for (int i = 0; i < D; ++i) Xd += ev[i] * g[d][i];
+                        if (a.s.limiter) {
+                            double dw = 0.0;
+#pragma unroll
+                            for (int i = 0; i < D; ++i) dw += F->d[side][i] * g[d][i];
+                            Xd *= venkata_phi(dw, fc[d], mn[d], mx[d], omega, sn >= VSMALL ? 1.0e-10 : VSMALL);
+                        }
                         // a boundary face's dead-band value 0.5 (f + f + X + X) is f + X
                         const double fbar = fc[d] + Xd;
                         const double eq = d == 0 ? gs : hs;

@@ -67,6 +67,11 @@ __device__ __forceinline__ T bnd_mix(const KArgs &a, const T *__restrict__ fd, i
         T gr[3];
         lsq_grad<D, T>(a, fd, c, v, fc, gr);
         T X = extrap<D, T>(a, f, 0, xh, gr);
+        if (a.s.limiter) {
+            double dw = 0.0;
+            for (int i = 0; i < D; ++i) dw += double(face_d<T>(a, f, 0, i)) * double(gr[i]);
+            X *= T(nb_phi<T>(a, fd, c, v, double(fc), dw, sn >= VSMALL ? 1.0e-10 : VSMALL));
+        }
         fb = (sn >= VSMALL || sn < -VSMALL) ? fc + X : T(0.5) * (fc + fc + X + X);
     }
     return lerp_store<T>(fb, eqv, mix);
@@ -168,8 +173,12 @@ __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restri
             gr[1] += a.nb_a[3 * j + 1] * df;
             gr[2] += a.nb_a[3 * j + 2] * df;
         }
-        const double X = (a.face_d[6 * f] - xh[0]) * gr[0] + (a.face_d[6 * f + 1] - xh[1]) * gr[1] +
-                         (a.face_d[6 * f + 2] - xh[2]) * gr[2];
+        double X = (a.face_d[6 * f] - xh[0]) * gr[0] + (a.face_d[6 * f + 1] - xh[1]) * gr[1] +
+                   (a.face_d[6 * f + 2] - xh[2]) * gr[2];
+        if (a.s.limiter) {
+            const double dw = a.face_d[6 * f] * gr[0] + a.face_d[6 * f + 1] * gr[1] + a.face_d[6 * f + 2] * gr[2];
+            X *= nb_phi<T>(a, fd, c, v, fs, dw, sn >= VSMALL ? 1.0e-10 : VSMALL);
+        }
         fb = dead ? 0.5 * (fs + fs + X + X) : fs + X;
     }
     if constexpr (sizeof(T) == 4) fb += a.fref_d[k * a.s.nv + v];
@@ -325,6 +334,23 @@ __device__ __forceinline__ const T *coef_ptr(const double *d, const float *f, si
     else return reinterpret_cast<const T *>(f + off);
 }
 
+// Limiter factor of one stencil group (the cell first, then its face
+// neighbours, padded with the cell) for the face offset d and gradient gr.
+template <int D, int GS, class T>
+__device__ __forceinline__ double group_phi(const T *vals, const T *d, const T *gr, double omega, double eps) {
+    double mn = double(vals[0]), mx = mn;
+#pragma unroll
+    for (int j = 1; j < GS; ++j) {
+        const double w = double(vals[j]);
+        mn = fmin(mn, w);
+        mx = fmax(mx, w);
+    }
+    double dw = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) dw += double(d[i]) * double(gr[i]);
+    return venkata_phi(dw, double(vals[0]), mn, mx, omega, eps);
+}
+
 // (1) raw moments of f-bar+ per (face, velocity tile); one wave per pair.
 // All stencil values of an iteration are loaded before any is used.
 template <int MODEL, int D, class T>
@@ -393,11 +419,22 @@ __global__ __launch_bounds__(BLOCK) void k_face_moments(KArgs a) {
                 if (SH) group_grad<D, MAXN, T>(vh + GS, av + 3 * MAXN, gn_h);
             }
         }
-        const T Xog = dot_ext<D, T>(d[0], d[1], d[2], xh, go_g), Xng = dot_ext<D, T>(d[3], d[4], d[5], xh, gn_g);
+        T Xog = dot_ext<D, T>(d[0], d[1], d[2], xh, go_g), Xng = dot_ext<D, T>(d[3], d[4], d[5], xh, gn_g);
+        double om_o = 0.0, om_n = 0.0;
+        if (SH && a.s.limiter) {  // cdugks_shakhov.cpp:231-266
+            om_o = venkata_omega(a.s.venkata_k, a.cell_vol[ids[0]], D);
+            om_n = venkata_omega(a.s.venkata_k, a.cell_vol[ids[GS]], D);
+            Xog *= T(group_phi<D, GS, T>(vg, d, go_g, om_o, up > 0 ? 1.0e-10 : VSMALL));
+            Xng *= T(group_phi<D, GS, T>(vg + GS, d + 3, gn_g, om_n, VSMALL));
+        }
         const T gb = up > 0 ? vg[0] + Xog : (up < 0 ? vg[GS] + Xng : T(0.5) * (vg[0] + vg[GS] + Xog + Xng));
         T hb = T(0);
         if (SH) {
-            const T Xoh = dot_ext<D, T>(d[0], d[1], d[2], xh, go_h), Xnh = dot_ext<D, T>(d[3], d[4], d[5], xh, gn_h);
+            T Xoh = dot_ext<D, T>(d[0], d[1], d[2], xh, go_h), Xnh = dot_ext<D, T>(d[3], d[4], d[5], xh, gn_h);
+            if (a.s.limiter) {
+                Xoh *= T(group_phi<D, GS, T>(vh, d, go_h, om_o, up > 0 ? 1.0e-10 : VSMALL));
+                Xnh *= T(group_phi<D, GS, T>(vh + GS, d + 3, gn_h, om_n, VSMALL));
+            }
             hb = up > 0 ? vh[0] + Xoh : (up < 0 ? vh[GS] + Xnh : T(0.5) * (vh[0] + vh[GS] + Xoh + Xnh));
         }
         const double X0 = X.x, X1 = X.y, X2 = X.z;
@@ -1056,6 +1093,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
             if (SH) group_grad<D, MAXN, T>(vh, sa, gch);
         }
         T Fg = T(0), Fh = T(0);
+        const double om_c = SH && a.s.limiter ? venkata_omega(a.s.venkata_k, a.cell_vol[c], D) : 0.0;
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
             if (k < nfc) {
@@ -1093,12 +1131,15 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
                 const bool bface = BND && a.cs_other[idx] < 0;
                 if (bface) {
                     // boundary face: owner == neighbour == c
-                    const T Xg = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gcg);
+                    T Xg = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gcg);
+                    const double eps_c = up > 0 ? 1.0e-10 : VSMALL;
+                    if (SH && a.s.limiter) Xg *= T(group_phi<D, GS, T>(vg, dc, gcg, om_c, eps_c));
                     const T bg = up != 0 ? fcg + Xg : T(0.5) * (fcg + fcg + Xg + Xg);
                     const T gmx = lerp_store<T>(bg, gs, mix);
                     T hmx = T(0);
                     if (SH) {
-                        const T Xh = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gch);
+                        T Xh = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gch);
+                        if (a.s.limiter) Xh *= T(group_phi<D, GS, T>(vh, dc, gch, om_c, eps_c));
                         const T bh = up != 0 ? fch + Xh : T(0.5) * (fch + fch + Xh + Xh);
                         hmx = lerp_store<T>(bh, hs, mix);
                     }
@@ -1121,8 +1162,17 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
                         if (SH) group_grad<D, MAXN, T>(vho, sa + (1 + k) * MAXN * 3, goh);
                     }
                     const T fog = vgo[0];
-                    const T Xcg = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gcg);
-                    const T Xog = dot_ext<D, T>(dn[0], dn[1], dn[2], xh, gog);
+                    T Xcg = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gcg);
+                    T Xog = dot_ext<D, T>(dn[0], dn[1], dn[2], xh, gog);
+                    // limiter: eps 1e-10 for the face's owner when it is upwind
+                    const double eps_c = own_c && up > 0 ? 1.0e-10 : VSMALL;
+                    const double eps_o = !own_c && up > 0 ? 1.0e-10 : VSMALL;
+                    double om_o = 0.0;
+                    if (SH && a.s.limiter) {
+                        om_o = venkata_omega(a.s.venkata_k, a.cell_vol[sid[(1 + k) * GS]], D);
+                        Xcg *= T(group_phi<D, GS, T>(vg, dc, gcg, om_c, eps_c));
+                        Xog *= T(group_phi<D, GS, T>(vgo, dn, gog, om_o, eps_o));
+                    }
                     // dead-band: 0.5 (f_own + f_nei + X_own + X_nei), owner first
                     const T bg = up_c ? fcg + Xcg
                                       : (up_o ? fog + Xog
@@ -1131,8 +1181,12 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
                     gv = lerp_store<T>(bg, gs, mix);
                     if (SH) {
                         const T foh = vho[0];
-                        const T Xch = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gch);
-                        const T Xoh = dot_ext<D, T>(dn[0], dn[1], dn[2], xh, goh);
+                        T Xch = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gch);
+                        T Xoh = dot_ext<D, T>(dn[0], dn[1], dn[2], xh, goh);
+                        if (a.s.limiter) {
+                            Xch *= T(group_phi<D, GS, T>(vh, dc, gch, om_c, eps_c));
+                            Xoh *= T(group_phi<D, GS, T>(vho, dn, goh, om_o, eps_o));
+                        }
                         const T bh = up_c ? fch + Xch
                                           : (up_o ? foh + Xoh
                                                   : (own_c ? T(0.5) * (fch + foh + Xch + Xoh)
@@ -1955,6 +2009,8 @@ void set_const(StepConst &s, const meso_step_params &p) {
     s.D = p.dim;
     s.K = p.K;
     s.grad = p.gradient_switch ? 1 : 0;
+    s.limiter = p.model == MESO_MODEL_CDUGKS_SHAKHOV && p.limiter_switch ? 1 : 0;
+    s.venkata_k = p.venkata_k;
     s.dt = p.dt;
     s.half_dt = 0.5 * p.dt;
     s.Pr = p.Pr;

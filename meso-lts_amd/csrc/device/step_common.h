@@ -160,7 +160,28 @@ struct StepConst {
     double dt, half_dt;
     double Pr, R, T0, mu0, omega, Cv, gamma;
     double RT, tau;    // BGK
+    int limiter;       // Shakhov: Venkatakrishnan limiter of the reconstruction
+    double venkata_k;  // its k (meso's venkata_k; 0 unless read at construction)
 };
+
+// Venkatakrishnan limiter factor of a cell for the face increment
+// dw = (x_f - x_c) . grad f (Solver::venkata_limiter, src/solver/limiter.cpp:7-31):
+// w the cell value, [mn, mx] the range over the cell and its face neighbours,
+// omega = (k V^(1/D))^3. eps is 1e-10 where meso's owner is upwind and VSMALL
+// otherwise (cdugks_shakhov.cpp:234, 247, 262-265).
+__host__ __device__ inline double venkata_phi(double dw, double w, double mn, double mx, double omega, double eps) {
+    if (fabs(dw) <= eps) return 1.0;
+    const double a = dw > 0.0 ? mx - w : mn - w;
+    const double aa = a * a, ab = a * dw, bb = dw * dw;
+    return (aa + 2.0 * ab + omega) / (aa + 2.0 * bb + ab + omega);
+}
+
+// omega of cell c (0 for k = 0, the value meso's limiter normally sees).
+__host__ __device__ inline double venkata_omega(double k, double vol, int D) {
+    if (k == 0.0) return 0.0;
+    const double kh = k * pow(vol, 1.0 / double(D));
+    return kh * kh * kh;
+}
 
 template <class R>
 __device__ __forceinline__ R dexp(R x);

@@ -228,26 +228,19 @@ __device__ __forceinline__ T extrap(const KArgs &a, int f, int side, const T xh[
     return r;
 }
 
-// f-bar+ of one distribution at face f, velocity v (cdugks_shakhov.cpp:223-298).
-template <int D, class T>
-__device__ __forceinline__ T fbar_at(const KArgs &a, const T *__restrict__ fd, int f, int own, int nei, double sn,
-                                     int v, const T xh[3]) {
+// Limiter factor of cell c (Shakhov limiter-switch) for the increment dw at
+// velocity v: the range of the values fd over c and its face neighbours.
+template <class T>
+__device__ __forceinline__ double nb_phi(const KArgs &a, const T *__restrict__ fd, int c, int v, double fc, double dw,
+                                         double eps) {
+    double mn = fc, mx = fc;
     const int nvp = a.s.nvp;
-    if (sn >= VSMALL || sn < -VSMALL) {
-        int up = (sn >= VSMALL) ? own : nei;
-        int side = (sn >= VSMALL) ? 0 : 1;
-        T fc = fd[size_t(up) * nvp + v];
-        if (!a.s.grad) return fc;
-        T gr[3];
-        lsq_grad<D, T>(a, fd, up, v, fc, gr);
-        return fc + extrap<D, T>(a, f, side, xh, gr);
+    for (int j = a.nb_ptr[c]; j < a.nb_ptr[c + 1]; ++j) {
+        const double w = double(fd[size_t(a.nb_ids[j]) * nvp + v]);
+        mn = fmin(mn, w);
+        mx = fmax(mx, w);
     }
-    T fo = fd[size_t(own) * nvp + v], fn = fd[size_t(nei) * nvp + v];
-    if (!a.s.grad) return T(0.5) * (fo + fn);
-    T go[3], gn[3];
-    lsq_grad<D, T>(a, fd, own, v, fo, go);
-    lsq_grad<D, T>(a, fd, nei, v, fn, gn);
-    return T(0.5) * (fo + fn + extrap<D, T>(a, f, 0, xh, go) + extrap<D, T>(a, f, 1, xh, gn));
+    return venkata_phi(dw, fc, mn, mx, venkata_omega(a.s.venkata_k, a.cell_vol[c], a.s.D), eps);
 }
 
 }  // namespace meso_dev

@@ -32,7 +32,8 @@ int model_from_solver(const std::string &s) {
 }
 
 void load_params(const Solver::Config &cfg, int model, int dim, double min_cell_size, double dvs_max_mag,
-                 meso_step_params &p) {
+                 meso_step_params &p, bool first) {
+    const double venkata_k = first ? 0.0 : p.venkata_k;  // kept across reloads, as meso's member
     std::memset(&p, 0, sizeof(p));
     p.model = model;
     p.dim = dim;
@@ -45,8 +46,12 @@ void load_params(const Solver::Config &cfg, int model, int dim, double min_cell_
         p.mu0 = cfg.get("ref-viscosity", 1.60e-05);
         p.omega = cfg.get("vhs-index", 0.81);
         p.gradient_switch = cfg.get("gradient-switch", true) ? 1 : 0;
-        if (cfg.get("limiter-switch", false))
-            throw std::invalid_argument("limiter-switch True is not supported by the HIP step");
+        p.limiter_switch = cfg.get("limiter-switch", false) ? 1 : 0;
+        // venkata_k is read only at construction with the gradient off and the
+        // limiter on (cdugks_shakhov.cpp:21-28); the limiter is applied only
+        // with the gradient on, so it normally sees 0
+        p.venkata_k = venkata_k;
+        if (first && !p.gradient_switch && p.limiter_switch) p.venkata_k = cfg.get("venkata-limiter-k", 1.0);
     } else {
         double RT = cfg.get("gas-constant", 0.5) * cfg.get("ref-temperature", 1.0);
         double L0 = cfg.get("ref-length", 1.0);
@@ -84,7 +89,7 @@ Case::Case(const std::string &path, double scale_override) : config(path) {
         double RT = config.get("gas-constant", 0.5) * config.get("ref-temperature", 1.0);
         dvs = DVS::gauss_hermite(dvs_file, D, RT);
     }
-    load_params(config, model, D, mesh.min_cell_size, dvs.max_magnitude, params);
+    load_params(config, model, D, mesh.min_cell_size, dvs.max_magnitude, params, true);
     build_arrays();
 }
 
@@ -289,7 +294,7 @@ int meso_case_reload_settings(meso_case *c) {
     try {
         cs->config.update_config(false);
         meso_host::load_params(cs->config, cs->model, cs->mesh.dimension(), cs->mesh.min_cell_size,
-                               cs->dvs.max_magnitude, cs->params);
+                               cs->dvs.max_magnitude, cs->params, false);
         return MESO_OK;
     } catch (const std::exception &e) {
         return fail(MESO_E_IO, e.what());

@@ -11,8 +11,10 @@ int model_from_solver(const std::string &solver);
 
 // Step parameters from [settings] (keys and defaults of cdugks_shakhov.cpp:7-33,
 // cdugks.cpp:9-18, dugks.cpp:9-15; DUGKS dt from CFL as dugks.cpp:47).
+// first: the solver's construction (meso reads venkata-limiter-k only then);
+// later calls re-read [settings] and keep venkata_k.
 void load_params(const MESO::Solver::Config &cfg, int model, int dim, double min_cell_size, double dvs_max_mag,
-                 meso_step_params &p);
+                 meso_step_params &p, bool first);
 
 struct Case {
     MESO::Solver::Config config;

@@ -50,7 +50,8 @@ class DvsDesc(C.Structure):
 class StepParams(C.Structure):
     _fields_ = [("model", C.c_int32), ("dim", C.c_int32), ("gradient_switch", C.c_int32), ("K", C.c_int32),
                 ("dt", C.c_double), ("Pr", C.c_double), ("R", C.c_double), ("T0", C.c_double), ("mu0", C.c_double),
-                ("omega", C.c_double), ("RT", C.c_double), ("tau", C.c_double)]
+                ("omega", C.c_double), ("RT", C.c_double), ("tau", C.c_double), ("limiter_switch", C.c_int32),
+                ("reserved", C.c_int32), ("venkata_k", C.c_double)]
 
 
 class StepReport(C.Structure):

@@ -290,7 +290,7 @@ def fmt(x: float) -> str:
 
 def shakhov_cavity_case(case_dir: str, mesh: GambitMesh, dvs: tuple, mu0: float, lid_u: float, dt: float | None = None,
                         cfl: float = 0.5, Pr: float = 2.0 / 3.0, max_step: int = 1000, residual_limit: float = 1e-7,
-                        name: str = "cavity", omega: float = 0.5, T_wall: float = 1.0) -> str:
+                        name: str = "cavity", omega: float = 0.5, T_wall: float = 1.0, limiter: bool = False) -> str:
     """Write mesh, DVS and a cdugks@shakhov case into case_dir; returns the case path.
     dvs = ("nc", degree, mount, scale) or ("sparse-gh", n, RT). dt defaults to
     cfl * min_cell_size / max|xi|."""
@@ -312,7 +312,7 @@ def shakhov_cavity_case(case_dir: str, mesh: GambitMesh, dvs: tuple, mu0: float,
     settings = {
         "solver": "cdugks@shakhov", "case-name": name, "mesh-file": "mesh.neu", "dvs-file": "dvs.dvs",
         "dvs-type": dvs_type, "write-interval": 100000, "max-step": max_step, "residual-limit": fmt(residual_limit),
-        "Pr": fmt(Pr), "gradient-switch": "True", "limiter-switch": "False", "timeStep": fmt(dt),
+        "Pr": fmt(Pr), "gradient-switch": "True", "limiter-switch": "True" if limiter else "False", "timeStep": fmt(dt),
         "gas-constant": "0.5", "gas-k": "0", "ref-density": "1.0", "ref-length": "1.0", "ref-temperature": "1.0",
         "ref-viscosity": fmt(mu0), "vhs-index": fmt(omega),
     }

@@ -147,8 +147,29 @@ struct oracle_ctx {
         }
     }
 
+    // Venkatakrishnan limiter of cell c for the increment dw = (x_f - x_c) . grad f
+    // (Solver::venkata_limiter, src/solver/limiter.cpp:7-31): min/max over the
+    // cell's face neighbours, omega = (k V^(1/D))^3.
+    double venkata(const double *fc, double dw, int c, double eps) const {
+        if (std::fabs(dw) <= eps) return 1.0;
+        const double kh = P.venkata_k * std::pow(cvol[c], 1.0 / double(D));
+        const double omega = kh * kh * kh;
+        const double wc = fc[c];
+        double wmax = wc, wmin = wc;
+        for (int j = nb_ptr[c]; j < nb_ptr[c + 1]; ++j) {
+            const double wn = fc[nb_ids[j]];
+            if (wn > wmax) wmax = wn;
+            if (wn < wmin) wmin = wn;
+        }
+        const double a = dw > 0.0 ? wmax - wc : wmin - wc;
+        const double aa = a * a, ab = a * dw, bb = dw * dw;
+        return (aa + 2.0 * ab + omega) / (aa + 2.0 * bb + ab + omega);
+    }
+
     // ---------------- face reconstruction (cdugks_shakhov.cpp:213-299, cdugks.cpp:124-153)
-    void reconstruct_faces(const std::vector<double> &cellv, std::vector<double> &facev) {
+    // limit: the Shakhov solver's limiter-switch (cdugks_shakhov.cpp:231-266;
+    // eps 1e-10 on the owner-upwind branch, VSMALL on the others)
+    void reconstruct_faces(const std::vector<double> &cellv, std::vector<double> &facev, bool limit = false) {
 #pragma omp parallel
         {
             std::vector<V3> gr;
@@ -162,7 +183,21 @@ struct oracle_ctx {
                     for (int f = 0; f < NF; ++f) {
                         int o = own[f], n = nei[f];
                         double nx = fn0[f] * x;
-                        if (nx >= VSMALL) {
+                        if (limit) {
+                            const V3 dro = fpos[f] - cpos[o], drn = fpos[f] - cpos[n];
+                            if (nx >= VSMALL) {
+                                const double phi = venkata(fc, dro * gr[o], o, 1.0e-10);
+                                ff[f] = fc[o] + phi * ((dro - x * half_dt) * gr[o]);
+                            } else if (nx < -VSMALL) {
+                                const double phi = venkata(fc, drn * gr[n], n, VSMALL);
+                                ff[f] = fc[n] + phi * ((drn - x * half_dt) * gr[n]);
+                            } else {
+                                const double po = venkata(fc, dro * gr[o], o, VSMALL);
+                                const double pn = venkata(fc, drn * gr[n], n, VSMALL);
+                                ff[f] = 0.5 * (fc[o] + fc[n] + po * ((dro - x * half_dt) * gr[o]) +
+                                               pn * ((drn - x * half_dt) * gr[n]));
+                            }
+                        } else if (nx >= VSMALL) {
                             ff[f] = fc[o] + ((fpos[f] - cpos[o]) - x * half_dt) * gr[o];
                         } else if (nx < -VSMALL) {
                             ff[f] = fc[n] + ((fpos[f] - cpos[n]) - x * half_dt) * gr[n];
@@ -192,8 +227,8 @@ struct oracle_ctx {
 
     // ---------------- Shakhov reconstruct (cdugks_shakhov.cpp:213-608)
     void reconstruct_shakhov() {
-        reconstruct_faces(g, gface);
-        reconstruct_faces(h, hface);
+        reconstruct_faces(g, gface, P.limiter_switch != 0);
+        reconstruct_faces(h, hface, P.limiter_switch != 0);
         // face macros + mixing (:300-363)
 #pragma omp parallel for schedule(static)
         for (int f = 0; f < NF; ++f) {

@@ -49,6 +49,18 @@ def case_factory(tmp_path_factory):
             p = mg.bgk_cavity_case(d, mg.hex_cavity(3, jitter=0.05, seed=1))
         elif name == "shak_quad6":
             p = mg.shakhov_cavity_case(d, mg.quad_cavity(6, jitter=0.1, seed=3), ("nc", 1, 7, 4.5), mu075, 0.2)
+        elif name == "shak_quad6_lim":  # the golden cases with the Venkatakrishnan limiter
+            p = mg.shakhov_cavity_case(d, mg.quad_cavity(6, jitter=0.1, seed=3), ("nc", 1, 7, 4.5), mu075, 0.2,
+                                       limiter=True)
+        elif name == "shak_tri6_lim":
+            p = mg.shakhov_cavity_case(d, mg.tri_cavity(6, jitter=0.2, seed=1), ("nc", 2, 4, 4.5), mu075, 0.2,
+                                       limiter=True)
+        elif name == "shak_hex3_lim":
+            p = mg.shakhov_cavity_case(d, mg.hex_cavity(3, jitter=0.05, seed=2), ("sparse-gh", 4, 0.5),
+                                       mg.viscosity_from_kn(0.1), 0.1, limiter=True)
+        elif name == "shak_quad6_nc6_lim":  # 36 velocities: the per-cell kernels
+            p = mg.shakhov_cavity_case(d, mg.quad_cavity(6, jitter=0.1, seed=3), ("nc", 1, 5, 4.5), mu075, 0.2,
+                                       limiter=True)
         elif name == "shak_quad12_nc24":
             p = mg.shakhov_cavity_case(d, mg.quad_cavity(12, jitter=0.1, seed=5), ("nc", 1, 23, 4.5), mu075, 0.2)
         elif name == "shak_tri6":
@@ -59,7 +71,8 @@ def case_factory(tmp_path_factory):
         elif name == "shak_kn1_quad10":
             p = mg.shakhov_cavity_case(d, mg.quad_cavity(10, jitter=0.1, seed=7), ("nc", 1, 16, 5.0),
                                        mg.viscosity_from_kn(1.0), 0.2)
-        elif name in ("shak_channel_pressure", "shak_channel_farfield", "shak_sym_slip", "bgk_channel_farfield"):
+        elif name in ("shak_channel_pressure", "shak_channel_farfield", "shak_sym_slip", "bgk_channel_farfield",
+                      "shak_sym_slip_lim"):
             p = channel_case(d, name)
         else:
             raise KeyError(name)
@@ -88,7 +101,10 @@ def split_sides(m, nx, ny):
 
 
 def channel_case(d, name):
-    """8x4 channel: left/right inlet+outlet, bottom wall, top wall / symmetry / slip."""
+    """8x4 channel: left/right inlet+outlet, bottom wall, top wall / symmetry / slip
+    (name suffix _lim: with the Venkatakrishnan limiter)."""
+    limiter = name.endswith("_lim")
+    name = name[:-4] if limiter else name
     os.makedirs(d, exist_ok=True)
     m = split_sides(mg.quad_cavity(8, 4, jitter=0.05, seed=11), 8, 4)
     mg.write_neu(m, os.path.join(d, "mesh.neu"))
@@ -108,7 +124,7 @@ def channel_case(d, name):
     settings = {"solver": "cdugks@shakhov", "mesh-file": "mesh.neu", "dvs-file": "dvs.dvs", "dvs-type": "Newton-Cotes",
                 "Pr": repr(2.0 / 3.0), "timeStep": "1.0e-3", "gas-constant": "0.5", "gas-k": "0",
                 "ref-temperature": "1.0", "ref-viscosity": repr(mg.viscosity_from_kn(0.1)), "vhs-index": "0.5",
-                "gradient-switch": "True", "limiter-switch": "False"}
+                "gradient-switch": "True", "limiter-switch": "True" if limiter else "False"}
     groups = [{"name": "fluid", "density": "fixedValue scalar 1.0", "temperature": "fixedValue scalar 1.0",
                "velocity": "fixedValue vector 0.05 0.0 0.0"}]
     wall = {"type": "wall", "velocity": "fixedValue vector 0.0 0.0 0.0", "temperature": "fixedValue scalar 1.0"}

@@ -51,6 +51,16 @@ def gen_inputs(name, d, ref_root):
     if name == "shak_hex3":
         return mg.shakhov_cavity_case(d, mg.hex_cavity(3, jitter=0.05, seed=2), ("sparse-gh", 4, 0.5),
                                       mg.viscosity_from_kn(0.1), 0.1)
+    # the same cavities with the Venkatakrishnan limiter (limiter-switch True)
+    if name == "shak_quad6_lim":
+        return mg.shakhov_cavity_case(d, mg.quad_cavity(6, jitter=0.1, seed=3), ("nc", 1, 7, 4.5), mu075, 0.2,
+                                      limiter=True)
+    if name == "shak_tri6_lim":
+        return mg.shakhov_cavity_case(d, mg.tri_cavity(6, jitter=0.2, seed=1), ("nc", 2, 4, 4.5), mu075, 0.2,
+                                      limiter=True)
+    if name == "shak_hex3_lim":
+        return mg.shakhov_cavity_case(d, mg.hex_cavity(3, jitter=0.05, seed=2), ("sparse-gh", 4, 0.5),
+                                      mg.viscosity_from_kn(0.1), 0.1, limiter=True)
     if name in ("shak_channel_pressure", "shak_channel_farfield", "bgk_channel_farfield"):
         import conftest
         return conftest.channel_case(d, name)
@@ -90,6 +100,9 @@ CASES = {
     "shak_quad6": ([1, 2, 10, 50], 2),
     "shak_tri6": ([1, 10], None),
     "shak_hex3": ([1, 5], None),
+    "shak_quad6_lim": ([1, 2, 10, 50], None),
+    "shak_tri6_lim": ([1, 10], None),
+    "shak_hex3_lim": ([1, 5], None),
     "shak_channel_pressure": ([1, 10], None),
     "shak_channel_farfield": ([1, 10], None),
     "bgk_channel_farfield": ([1, 10], None),

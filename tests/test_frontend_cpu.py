@@ -187,9 +187,17 @@ def test_case_errors(tmp_path):
         (vd / "case.txt").write_text(text.replace(old, new))
         return str(vd / "case.txt")
 
-    # meso refuses limiter-switch True in the solver constructors we replace
-    with pytest.raises(capi.MesoError):
-        capi.Case(variant("lim", "limiter-switch       False", "limiter-switch       True"))
+    # the limiter switch; venkata_k stays 0 with the gradient on (meso reads
+    # venkata-limiter-k only with gradient-switch False, cdugks_shakhov.cpp:21-28)
+    lim = capi.Case(variant("lim", "limiter-switch       False", "limiter-switch       True"))
+    assert lim.params.limiter_switch == 1 and lim.params.venkata_k == 0.0
+    lim.reload_settings()
+    assert lim.params.limiter_switch == 1 and lim.params.venkata_k == 0.0
+    vk = capi.Case(variant("limk", "gradient-switch      True\nlimiter-switch       False",
+                           "gradient-switch      False\nlimiter-switch       True\nvenkata-limiter-k    2.5"))
+    assert vk.params.venkata_k == 2.5 and vk.params.gradient_switch == 0
+    vk.reload_settings()
+    assert vk.params.venkata_k == 2.5
     # unknown solver
     with pytest.raises(capi.MesoError):
         capi.Case(variant("solver", "cdugks@shakhov", "cdugks@nosuch"))
