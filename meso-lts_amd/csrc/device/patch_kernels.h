@@ -16,6 +16,10 @@
 // f - f_ref. The equilibria of FP32 storage are evaluated in FP32 in
 // deviation form (their weight in one step is dt/tau or C_s).
 //
+// LIM: the Venkatakrishnan limiter of the reconstruction (Shakhov
+// limiter-switch); a template argument, so that the kernels without it keep
+// their registers.
+//
 // Upwind dead-band (|xi.n| < 1e-30): the moment kernel forms meso's average of
 // both sides; the update kernel gives the face no flux, since the flux term is
 // (xi.n) A f with |xi.n| < 1e-30.
@@ -224,21 +228,25 @@ struct UpdateRecs<double> {
     __device__ static EqParams face_eq(const KArgs &a, int f) { return a.face_eq[f]; }
 };
 
-// f_ref flux factor sum(+-A n) of a cell as the update kernel keeps it.
+// f_ref flux factor sum(+-A n) of a cell as the update kernel keeps it
+// (8-byte aligned: it follows records of 8-byte multiples in LDS).
+struct CellS2 {
+    double x, y;
+};
 struct CellS3 {
-    float x, y, z, pad;
+    double x, y, z, pad;
 };
 template <int D>
-using CellS = typename std::conditional<D == 2, float2, CellS3>::type;
+using CellS = typename std::conditional<D == 2, CellS2, CellS3>::type;
 
 template <int D>
-__device__ __forceinline__ CellS<D> cell_s(const float *p) {
+__device__ __forceinline__ CellS<D> cell_s(const double *p) {
     CellS<D> r;
     r.x = p[0];
     r.y = p[1];
     if constexpr (D == 3) {
         r.z = p[2];
-        r.pad = 0.0f;
+        r.pad = 0.0;
     }
     return r;
 }
@@ -246,16 +254,17 @@ __device__ __forceinline__ CellS<D> cell_s(const float *p) {
 // xi . sum(+-A n) of a cell
 template <int D>
 __device__ __forceinline__ double cell_xs(const CellS<D> &s, const double *xv) {
-    double r = xv[0] * double(s.x) + xv[1] * double(s.y);
-    if constexpr (D == 3) r += xv[2] * double(s.z);
+    double r = xv[0] * s.x + xv[1] * s.y;
+    if constexpr (D == 3) r += xv[2] * s.z;
     return r;
 }
 
-// Bytes of LDS the face and cell records of k_patch_update take.
+// Bytes of LDS the face and cell records of k_patch_update take (the f_ref
+// flux factor only for FP32 storage, which stores deviations from f_ref).
 template <class S, int D>
 __host__ __device__ constexpr size_t update_rec_bytes(int maxf, int maxo) {
     return size_t(maxf) * (sizeof(typename UpdateRecs<S>::Eq) + 8) +
-           size_t(maxo) * (sizeof(CellRec<S, D>) + sizeof(CellS<D>));
+           size_t(maxo) * (sizeof(CellRec<S, D>) + (sizeof(S) == 4 ? sizeof(CellS<D>) : 0));
 }
 
 // Equilibrium pair of face f at the lane velocity from its record e, in the
@@ -326,7 +335,7 @@ __device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn,
 // ------------------------------------------------------------- moments
 // Raw moments of f-bar+ per face of the moment patches, summed per velocity
 // chunk into part_f[face][chunk][col] (the layout k_face_eq reads).
-template <int MODEL, class S, int VT, int D>
+template <int MODEL, class S, int VT, int D, bool LIM>
 __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.bad_count > 0) {
@@ -412,9 +421,9 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
 #pragma unroll
                 for (int k = 0; k < D; ++k) g[d][k] = 0.0;
                 if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d]);
-                if (a.s.limiter) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d]);
+                if constexpr (LIM) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d]);
             }
-            const double omega = a.s.limiter ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
+            const double omega = LIM ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
             const double xv[3] = {X.x, X.y, X.z};
 #pragma unroll
             for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
@@ -438,7 +447,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                     double Xd = 0.0;
 #pragma unroll
                     for (int i = 0; i < D; ++i) Xd += ev[i] * g[d][i];
-                    if (a.s.limiter) {
+                    if constexpr (LIM) {
                         double dw = 0.0;
 #pragma unroll
                         for (int i = 0; i < D; ++i) dw += F->d[side][i] * g[d][i];
@@ -514,7 +523,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
 // the patch faces and the update constants of the owned cells are copied to
 // LDS once per block (read per lane from HBM in every tile they would cost
 // address arithmetic and L2 latency).
-template <int MODEL, class S, int VT, bool TAB, int D>
+template <int MODEL, class S, int VT, bool TAB, int D, bool LIM>
 __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.halt) return;
@@ -569,7 +578,7 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     for (int r = threadIdx.x; r < nown; r += BLOCKP) {
         const int c = up_items[i0 + r].cell;
         crl[r] = reinterpret_cast<const CellRec<S, D> *>(a.crec)[c];
-        csl[r] = cell_s<D>(a.cell_S + 3 * c);
+        if constexpr (sizeof(S) == 4) csl[r] = cell_s<D>(a.cell_S + 3 * c);
     }
     const int ntiles = (a.s.nv + VT - 1) / VT;
     const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
@@ -592,9 +601,9 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
 #pragma unroll
                 for (int k = 0; k < D; ++k) g[d][k] = 0.0;
                 if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d]);
-                if (a.s.limiter) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d]);
+                if constexpr (LIM) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d]);
             }
-            const double omega = a.s.limiter ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
+            const double omega = LIM ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
 #pragma unroll
             for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
                 if (k >= nface) continue;
@@ -625,7 +634,7 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                         double Xd = 0.0;
 #pragma unroll
                         for (int i = 0; i < D; ++i) Xd += ev[i] * g[d][i];
-                        if (a.s.limiter) {
+                        if constexpr (LIM) {
                             double dw = 0.0;
 #pragma unroll
                             for (int i = 0; i < D; ++i) dw += F->d[side][i] * g[d][i];

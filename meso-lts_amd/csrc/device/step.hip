@@ -56,7 +56,7 @@ __device__ __forceinline__ double m2_at(const double *M, int i, int j) {
 
 // Value of face f at velocity v for its owner-side reconstruction on a
 // boundary face (own == nei == c), mixed with the face equilibrium.
-template <int D, class T>
+template <int D, class T, bool LIM>
 __device__ __forceinline__ T bnd_mix(const KArgs &a, const T *__restrict__ fd, int f, int c, double sn, int v,
                                      const T xh[3], T eqv, T mix) {
     const T fc = fd[size_t(c) * a.s.nvp + v];
@@ -67,7 +67,7 @@ __device__ __forceinline__ T bnd_mix(const KArgs &a, const T *__restrict__ fd, i
         T gr[3];
         lsq_grad<D, T>(a, fd, c, v, fc, gr);
         T X = extrap<D, T>(a, f, 0, xh, gr);
-        if (a.s.limiter) {
+        if constexpr (LIM) {
             double dw = 0.0;
             for (int i = 0; i < D; ++i) dw += double(face_d<T>(a, f, 0, i)) * double(gr[i]);
             X *= T(nb_phi<T>(a, fd, c, v, double(fc), dw, sn >= VSMALL ? 1.0e-10 : VSMALL));
@@ -78,7 +78,7 @@ __device__ __forceinline__ T bnd_mix(const KArgs &a, const T *__restrict__ fd, i
 }
 
 // Face-mixed pair (g, h) of boundary face f at velocity v.
-template <int MODEL, int D, class T>
+template <int MODEL, int D, class T, bool LIM>
 __device__ __forceinline__ void bnd_mix_pair(const KArgs &a, int f, int c, int v, const EqParams &ef, T &g, T &h) {
     T x[3];
     load_xi<T>(a, v, x);
@@ -88,12 +88,12 @@ __device__ __forceinline__ void bnd_mix_pair(const KArgs &a, int f, int c, int v
     T mix = T(ef.mix);
     T gs, hs;
     eq_store<MODEL, D, T>(a, ef, x, v, gs, hs);
-    g = bnd_mix<D, T>(a, (const T *)a.fin_g, f, c, sn, v, xh, gs, mix);
-    h = MODEL == CDUGKS_SHAKHOV ? bnd_mix<D, T>(a, (const T *)a.fin_h, f, c, sn, v, xh, hs, mix) : T(0);
+    g = bnd_mix<D, T, LIM>(a, (const T *)a.fin_g, f, c, sn, v, xh, gs, mix);
+    h = MODEL == CDUGKS_SHAKHOV ? bnd_mix<D, T, LIM>(a, (const T *)a.fin_h, f, c, sn, v, xh, hs, mix) : T(0);
 }
 
 // Final boundary value at velocity v given the mixed value (gm_, hm_) there.
-template <int MODEL, int D, class T>
+template <int MODEL, int D, class T, bool LIM>
 __device__ __forceinline__ void bnd_value(const KArgs &a, const BndParams &P, int f, int c, int v, double sn,
                                           const EqParams &ef, T gmix, T hmix, T &g, T &h) {
     g = gmix;
@@ -127,7 +127,7 @@ __device__ __forceinline__ void bnd_value(const KArgs &a, const BndParams &P, in
         case BC_SYMMETRY: {
             int vm = a.mirror[3 * v + P.dir];
             T gr, hr;
-            bnd_mix_pair<MODEL, D, T>(a, f, c, vm, ef, gr, hr);
+            bnd_mix_pair<MODEL, D, T, LIM>(a, f, c, vm, ef, gr, hr);
             T gv, hv;
             if (P.kind == BC_SLIP_WALL) {
                 gv = (T(1) - T(P.alpha_u)) * gi + T(P.alpha_u) * gr;
@@ -156,7 +156,7 @@ __device__ __forceinline__ void bnd_value(const KArgs &a, const BndParams &P, in
 // deviation of FP32 storage plus f_ref). k_boundary uses them for the wall
 // density sums and the exact boundary flux moments, so the conservative update
 // of boundary cells carries no FP32 rounding of weights or geometry.
-template <class T>
+template <class T, bool LIM>
 __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restrict__ fd, int k, int f, int c,
                                                double sn, int v, const double xh[3], double eq_full, double mix) {
     const int nvp = a.s.nvp;
@@ -175,7 +175,7 @@ __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restri
         }
         double X = (a.face_d[6 * f] - xh[0]) * gr[0] + (a.face_d[6 * f + 1] - xh[1]) * gr[1] +
                    (a.face_d[6 * f + 2] - xh[2]) * gr[2];
-        if (a.s.limiter) {
+        if constexpr (LIM) {
             const double dw = a.face_d[6 * f] * gr[0] + a.face_d[6 * f + 1] * gr[1] + a.face_d[6 * f + 2] * gr[2];
             X *= nb_phi<T>(a, fd, c, v, fs, dw, sn >= VSMALL ? 1.0e-10 : VSMALL);
         }
@@ -185,7 +185,7 @@ __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restri
     return (1.0 - mix) * fb + mix * eq_full;
 }
 
-template <int MODEL, int D, class T>
+template <int MODEL, int D, class T, bool LIM>
 __device__ __forceinline__ void bnd_mix_pair_full(const KArgs &a, int f, int c, int v, const EqParams &ef, double &g,
                                                   double &h) {
     const double x[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
@@ -195,12 +195,12 @@ __device__ __forceinline__ void bnd_mix_pair_full(const KArgs &a, int f, int c, 
     double gs, hs = 0.0;
     if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<double>(ef, x, D, a.s.K, gs, hs);
     else gs = bgk_eq<double>(ef, x);
-    g = bnd_mix_full<T>(a, (const T *)a.fin_g, 0, f, c, sn, v, xh, gs, ef.mix);
-    h = MODEL == CDUGKS_SHAKHOV ? bnd_mix_full<T>(a, (const T *)a.fin_h, 1, f, c, sn, v, xh, hs, ef.mix) : 0.0;
+    g = bnd_mix_full<T, LIM>(a, (const T *)a.fin_g, 0, f, c, sn, v, xh, gs, ef.mix);
+    h = MODEL == CDUGKS_SHAKHOV ? bnd_mix_full<T, LIM>(a, (const T *)a.fin_h, 1, f, c, sn, v, xh, hs, ef.mix) : 0.0;
 }
 
 // Final full value at velocity v of boundary face f from the mixed value there.
-template <int MODEL, int D, class T>
+template <int MODEL, int D, class T, bool LIM>
 __device__ __forceinline__ void bnd_value_full(const KArgs &a, const BndParams &P, int f, int c, int v, double sn,
                                                const EqParams &ef, double gmix, double hmix, double &g, double &h) {
     g = gmix;
@@ -236,7 +236,7 @@ __device__ __forceinline__ void bnd_value_full(const KArgs &a, const BndParams &
         case BC_SYMMETRY: {
             const int vm = a.mirror[3 * v + P.dir];
             double gr, hr;
-            bnd_mix_pair_full<MODEL, D, T>(a, f, c, vm, ef, gr, hr);
+            bnd_mix_pair_full<MODEL, D, T, LIM>(a, f, c, vm, ef, gr, hr);
             double gv, hv;
             if (P.kind == BC_SLIP_WALL) {
                 gv = (1.0 - P.alpha_u) * gi + P.alpha_u * gr;
@@ -353,7 +353,7 @@ __device__ __forceinline__ double group_phi(const T *vals, const T *d, const T *
 
 // (1) raw moments of f-bar+ per (face, velocity tile); one wave per pair.
 // All stencil values of an iteration are loaded before any is used.
-template <int MODEL, int D, class T>
+template <int MODEL, int D, class T, bool LIM>
 __global__ __launch_bounds__(BLOCK) void k_face_moments(KArgs a) {
     if (*a.bad_count > 0) {
         if (blockIdx.x == 0 && threadIdx.x == 0) *a.halt = 1;
@@ -421,7 +421,7 @@ __global__ __launch_bounds__(BLOCK) void k_face_moments(KArgs a) {
         }
         T Xog = dot_ext<D, T>(d[0], d[1], d[2], xh, go_g), Xng = dot_ext<D, T>(d[3], d[4], d[5], xh, gn_g);
         double om_o = 0.0, om_n = 0.0;
-        if (SH && a.s.limiter) {  // cdugks_shakhov.cpp:231-266
+        if constexpr (SH && LIM) {  // cdugks_shakhov.cpp:231-266
             om_o = venkata_omega(a.s.venkata_k, a.cell_vol[ids[0]], D);
             om_n = venkata_omega(a.s.venkata_k, a.cell_vol[ids[GS]], D);
             Xog *= T(group_phi<D, GS, T>(vg, d, go_g, om_o, up > 0 ? 1.0e-10 : VSMALL));
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(BLOCK) void k_face_moments(KArgs a) {
         T hb = T(0);
         if (SH) {
             T Xoh = dot_ext<D, T>(d[0], d[1], d[2], xh, go_h), Xnh = dot_ext<D, T>(d[3], d[4], d[5], xh, gn_h);
-            if (a.s.limiter) {
+            if constexpr (LIM) {
                 Xoh *= T(group_phi<D, GS, T>(vh, d, go_h, om_o, up > 0 ? 1.0e-10 : VSMALL));
                 Xnh *= T(group_phi<D, GS, T>(vh + GS, d + 3, gn_h, om_n, VSMALL));
             }
@@ -754,7 +754,7 @@ __device__ __forceinline__ void block_sum(double (&x)[N], double *lds /* [N * BL
 }
 
 // (3) boundary faces: one block per face
-template <int MODEL, int D, class T>
+template <int MODEL, int D, class T, bool LIM>
 __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
     if (*a.halt) return;
     constexpr int NFL = D + 2;
@@ -827,7 +827,7 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
                     s[1] += kn * a.w[v] * gm;
                 } else {
                     double gmx, hmx;
-                    bnd_mix_pair_full<MODEL, D, T>(a, f, c, v, ef, gmx, hmx);
+                    bnd_mix_pair_full<MODEL, D, T, LIM>(a, f, c, v, ef, gmx, hmx);
                     s[0] -= kn * a.w[v] * gmx;
                 }
             }
@@ -849,9 +849,9 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
     for (int k = 0; k < NFL; ++k) acc[k] = 0.0;
     for (int v = threadIdx.x; v < nv; v += BLOCK) {
         double gmx, hmx, g, h;
-        bnd_mix_pair_full<MODEL, D, T>(a, f, c, v, ef, gmx, hmx);
+        bnd_mix_pair_full<MODEL, D, T, LIM>(a, f, c, v, ef, gmx, hmx);
         const double sn = ndot(a, f, v);
-        bnd_value_full<MODEL, D, T>(a, P, f, c, v, sn, ef, gmx, hmx, g, h);
+        bnd_value_full<MODEL, D, T, LIM>(a, P, f, c, v, sn, ef, gmx, hmx, g, h);
         const double X0 = a.xi[3 * v], X1 = a.xi[3 * v + 1], X2 = a.xi[3 * v + 2];
         const double wsn = a.w[v] * sn;
         const double wg = wsn * g;
@@ -982,7 +982,7 @@ __device__ __forceinline__ void cell_eq_store(const KArgs &a, const EqParams &e,
 // (4) cell flux + update; one wave per (cell, velocity tile). BND selects the
 // cells that have a boundary face (launched separately, so the interior
 // kernel carries no boundary code). TAB: tensor-product DVS with axis tables.
-template <int MODEL, int D, class T, bool TAB, bool BND>
+template <int MODEL, int D, class T, bool TAB, bool BND, bool LIM>
 __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, int list_count) {
     extern __shared__ unsigned char lds_raw[];
     if (*a.halt) return;
@@ -1093,7 +1093,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
             if (SH) group_grad<D, MAXN, T>(vh, sa, gch);
         }
         T Fg = T(0), Fh = T(0);
-        const double om_c = SH && a.s.limiter ? venkata_omega(a.s.venkata_k, a.cell_vol[c], D) : 0.0;
+        const double om_c = SH && LIM ? venkata_omega(a.s.venkata_k, a.cell_vol[c], D) : 0.0;
 #pragma unroll
         for (int k = 0; k < MAXF; ++k) {
             if (k < nfc) {
@@ -1133,19 +1133,20 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
                     // boundary face: owner == neighbour == c
                     T Xg = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gcg);
                     const double eps_c = up > 0 ? 1.0e-10 : VSMALL;
-                    if (SH && a.s.limiter) Xg *= T(group_phi<D, GS, T>(vg, dc, gcg, om_c, eps_c));
+                    if constexpr (SH && LIM) Xg *= T(group_phi<D, GS, T>(vg, dc, gcg, om_c, eps_c));
                     const T bg = up != 0 ? fcg + Xg : T(0.5) * (fcg + fcg + Xg + Xg);
                     const T gmx = lerp_store<T>(bg, gs, mix);
                     T hmx = T(0);
                     if (SH) {
                         T Xh = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gch);
-                        if (a.s.limiter) Xh *= T(group_phi<D, GS, T>(vh, dc, gch, om_c, eps_c));
+                        if constexpr (LIM) Xh *= T(group_phi<D, GS, T>(vh, dc, gch, om_c, eps_c));
                         const T bh = up != 0 ? fch + Xh : T(0.5) * (fch + fch + Xh + Xh);
                         hmx = lerp_store<T>(bh, hs, mix);
                     }
                     const int bi = a.cs_bnd[idx];
                     if (bi >= 0) {
-                        bnd_value<MODEL, D, T>(a, a.bnd[bi], f, c, v, double(sn), a.face_eq[f], gmx, hmx, gv, hv);
+                        bnd_value<MODEL, D, T, LIM>(a, a.bnd[bi], f, c, v, double(sn), a.face_eq[f], gmx, hmx, gv,
+                                                    hv);
                     } else {
                         gv = gmx;
                         hv = hmx;
@@ -1168,7 +1169,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
                     const double eps_c = own_c && up > 0 ? 1.0e-10 : VSMALL;
                     const double eps_o = !own_c && up > 0 ? 1.0e-10 : VSMALL;
                     double om_o = 0.0;
-                    if (SH && a.s.limiter) {
+                    if constexpr (SH && LIM) {
                         om_o = venkata_omega(a.s.venkata_k, a.cell_vol[sid[(1 + k) * GS]], D);
                         Xcg *= T(group_phi<D, GS, T>(vg, dc, gcg, om_c, eps_c));
                         Xog *= T(group_phi<D, GS, T>(vgo, dn, gog, om_o, eps_o));
@@ -1183,7 +1184,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
                         const T foh = vho[0];
                         T Xch = dot_ext<D, T>(dc[0], dc[1], dc[2], xh, gch);
                         T Xoh = dot_ext<D, T>(dn[0], dn[1], dn[2], xh, goh);
-                        if (a.s.limiter) {
+                        if constexpr (LIM) {
                             Xch *= T(group_phi<D, GS, T>(vh, dc, gch, om_c, eps_c));
                             Xoh *= T(group_phi<D, GS, T>(vho, dn, goh, om_o, eps_o));
                         }
@@ -1201,10 +1202,10 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
         }
         if constexpr (sizeof(T) == 4) {
             // flux of f_ref itself: f_ref xi . sum(+-A n); non-zero where faces are not planar
-            const float xs = x[0] * a.cell_S[3 * c] + x[1] * a.cell_S[3 * c + 1] + x[2] * a.cell_S[3 * c + 2];
-            const float2 fr = a.fref_f2[v];
-            Fg += fr.x * xs;
-            if (SH) Fh += fr.y * xs;
+            const double xs = X.x * a.cell_S[3 * c] + X.y * a.cell_S[3 * c + 1] + X.z * a.cell_S[3 * c + 2];
+            const double2 fr = a.fref_d2[v];
+            Fg += T(fr.x * xs);
+            if (SH) Fh += T(fr.y * xs);
         }
         const double X0 = X.x, X1 = X.y, X2 = X.z;
         const double wv = X.w;
@@ -1530,7 +1531,8 @@ struct meso_hip_ctx {
     // device pointers
     int2 *face_cells = nullptr;
     double *face_n = nullptr, *face_d = nullptr, *cf_sA = nullptr, *nb_a = nullptr, *cell_vol = nullptr;
-    float *face_df = nullptr, *nb_af = nullptr, *xif = nullptr, *cell_S = nullptr;
+    float *face_df = nullptr, *nb_af = nullptr, *xif = nullptr;
+    double *cell_S = nullptr;
     int *cs_nf = nullptr, *cs_face = nullptr, *cs_other = nullptr, *cs_own = nullptr, *cs_bnd = nullptr,
         *lsq_cnt = nullptr, *lsq_nb = nullptr;
     double *cs_sA = nullptr, *cs_d = nullptr, *lsq_a = nullptr;
@@ -1557,8 +1559,8 @@ struct meso_hip_ctx {
         mp_maxr = 0;
     int p_chunks = 0, p_vchunk = 0;
     size_t lds_up = 0, lds_mp = 0;
-    bool upd_attr = false;    // the dynamic-LDS limit of k_patch_update has been raised
-    bool mp_attr = false;     // the same for k_patch_moments
+    bool upd_attr[2] = {false, false};  // the dynamic-LDS limit of k_patch_update<.., LIM> has been raised
+    bool mp_attr[2] = {false, false};   // the same for k_patch_moments
     const void *up_items = nullptr, *mp_items = nullptr;  // meso_patch::ItemT<D>
     const void *up_faces = nullptr, *mp_faces = nullptr;  // meso_patch::FaceT<D>
     int *up_item_ptr = nullptr, *up_face_ptr = nullptr, *up_nown = nullptr, *mp_item_ptr = nullptr,
@@ -1884,7 +1886,7 @@ struct Launch {
 
 // Step with the LDS-staged patch kernels for the face moments and the
 // interior cells; boundary faces and mirror-boundary cells keep their kernels.
-template <int MODEL, class T, bool TAB, int D>
+template <int MODEL, class T, bool TAB, int D, bool LIM>
 int step_patch(meso_hip_ctx *c, KArgs a) {
     const StepConst &s = c->s;
     constexpr int WPB = BLOCK / 64;
@@ -1892,51 +1894,52 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
     constexpr int VTU = sizeof(T) == 4 ? 32 : 16;  // velocities per tile: update
     constexpr int MAXF = D == 2 ? 4 : 6;
     const int fwblk = (s.nf + WPB - 1) / WPB;
-    if (!c->mp_attr && c->lds_mp > 65536) {
-        HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D>),
+    if (!c->mp_attr[LIM] && c->lds_mp > 65536) {
+        HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_mp)));
-        c->mp_attr = true;
+        c->mp_attr[LIM] = true;
     }
-    LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D>), c->n_mp * c->p_chunks, c->lds_mp, a);
+    LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM>), c->n_mp * c->p_chunks, c->lds_mp, a);
     LAUNCH_SH(c, "face_eq", (k_face_eq<MODEL, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
-    if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, D, T>), s.nb, 0, a);
+    if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, D, T, LIM>), s.nb, 0, a);
     if (c->n_int > 0) {
         LAUNCH(c, "cell_prep", (k_cell_prep<MODEL, T, TAB, D>), (c->n_int + WPB - 1) / WPB, a);
-        if (!c->upd_attr && c->lds_up > 65536) {
-            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_update<MODEL, T, VTU, TAB, D>),
+        if (!c->upd_attr[LIM] && c->lds_up > 65536) {
+            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_update<MODEL, T, VTU, TAB, D, LIM>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_up)));
-            c->upd_attr = true;
+            c->upd_attr[LIM] = true;
         }
-        LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, D>), c->n_up * c->p_chunks, c->lds_up, a);
+        LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, D, LIM>), c->n_up * c->p_chunks, c->lds_up,
+                 a);
     }
     const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
     if (c->n_bcell > 0)
-        LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, D, T, TAB, true>),
+        LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, D, T, TAB, true, LIM>),
                   ((c->n_bcell + WPB - 1) / WPB) * s.ntile_c, lds, a, c->n_int, c->n_bcell);
     if (MODEL == CDUGKS_SHAKHOV) LAUNCH(c, "cell_finish", (k_cell_finish<MODEL>), (s.nc + BLOCK - 1) / BLOCK, a);
     HIP_TRY(c, hipGetLastError());
     return MESO_OK;
 }
 
-template <int MODEL, int D, class T, bool TAB>
-int step_impl(meso_hip_ctx *c, KArgs a) {
+template <int MODEL, int D, class T, bool TAB, bool LIM>
+int step_body(meso_hip_ctx *c, KArgs a) {
     const StepConst &s = c->s;
     constexpr int WPB = BLOCK / 64;
     constexpr int MF = MODEL == CDUGKS_SHAKHOV ? CDUGKS_SHAKHOV : CDUGKS_BGK;
     constexpr int MAXF = D == 2 ? 4 : 6;
     if constexpr (MODEL != DUGKS_BGK) {
-        if (c->patches) return step_patch<MODEL, T, TAB, D>(c, a);
+        if (c->patches) return step_patch<MODEL, T, TAB, D, LIM>(c, a);
     }
     const int fwblk = (s.nf + WPB - 1) / WPB;
-    LAUNCH_SH(c, "face_moments", (k_face_moments<MF, D, T>), fwblk * s.ntile_f, 0, a);
+    LAUNCH_SH(c, "face_moments", (k_face_moments<MF, D, T, LIM>), fwblk * s.ntile_f, 0, a);
     LAUNCH_SH(c, "face_eq", (k_face_eq<MF, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
-    if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MF, D, T>), s.nb, 0, a);
+    if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MF, D, T, LIM>), s.nb, 0, a);
     const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
     if (c->n_int > 0)
-        LAUNCH_SH(c, "cell_update", (k_cell_update<MODEL, D, T, TAB, false>),
+        LAUNCH_SH(c, "cell_update", (k_cell_update<MODEL, D, T, TAB, false, LIM>),
                   ((c->n_int + WPB - 1) / WPB) * s.ntile_c, lds, a, 0, c->n_int);
     if (c->n_bcell > 0)
-        LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, D, T, TAB, true>),
+        LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, D, T, TAB, true, LIM>),
                   ((c->n_bcell + WPB - 1) / WPB) * s.ntile_c, lds, a, c->n_int, c->n_bcell);
     if (MODEL == CDUGKS_SHAKHOV) LAUNCH(c, "cell_finish", (k_cell_finish<MODEL>), (s.nc + BLOCK - 1) / BLOCK, a);
     if (MODEL == DUGKS_BGK) {
@@ -1945,6 +1948,15 @@ int step_impl(meso_hip_ctx *c, KArgs a) {
     }
     HIP_TRY(c, hipGetLastError());
     return MESO_OK;
+}
+
+// The limiter (Shakhov only) selects kernels compiled with it.
+template <int MODEL, int D, class T, bool TAB>
+int step_impl(meso_hip_ctx *c, KArgs a) {
+    if constexpr (MODEL == CDUGKS_SHAKHOV) {
+        if (c->s.limiter) return step_body<MODEL, D, T, TAB, true>(c, a);
+    }
+    return step_body<MODEL, D, T, TAB, false>(c, a);
 }
 
 int step_dispatch(meso_hip_ctx *c, KArgs a) {
@@ -2217,7 +2229,10 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             return fail(set_err(nullptr, MESO_E_INVALID, "neighbour index out of range"));
     std::vector<int2> fc(nf);
     std::vector<double> fd(6 * size_t(nf)), sA(m->n_cell_faces), na(3 * size_t(m->n_nbr));
-    std::vector<float> fdf(6 * size_t(nf)), naf(3 * size_t(m->n_nbr)), xif(3 * size_t(nv)), cS(3 * size_t(nc), 0.f);
+    std::vector<float> fdf(6 * size_t(nf)), naf(3 * size_t(m->n_nbr)), xif(3 * size_t(nv));
+    // FP64: with non-planar (warped hexahedron) faces |S| / V is O(jitter / h), and
+    // the f_ref flux it carries is as large as the deviation fluxes
+    std::vector<double> cS(3 * size_t(nc), 0.0);
     for (int f = 0; f < nf; ++f) {
         fc[f] = make_int2(m->face_cells[2 * f], m->face_cells[2 * f + 1]);
         for (int side = 0; side < 2; ++side) {
@@ -2237,7 +2252,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         double S3[3] = {0.0, 0.0, 0.0};
         for (int k = m->cell_face_ptr[cc]; k < m->cell_face_ptr[cc + 1]; ++k)
             for (int r = 0; r < 3; ++r) S3[r] += sA[k] * m->face_normal[3 * size_t(m->cell_face_ids[k]) + r];
-        for (int r = 0; r < 3; ++r) cS[3 * size_t(cc) + r] = float(S3[r]);
+        for (int r = 0; r < 3; ++r) cS[3 * size_t(cc) + r] = S3[r];
         const double *C = m->lsq_C + 9 * size_t(cc);
         for (int j = m->nbr_ptr[cc]; j < m->nbr_ptr[cc + 1]; ++j) {
             const double *dr = m->lsq_dr + 3 * size_t(j);

@@ -44,7 +44,7 @@ struct KArgs {
     const double *nb_a;         // [nnz*3] w_j (Cx.dr_j, Cy.dr_j, Cz.dr_j)
     const float *nb_af;
     const double *cell_vol;
-    const float *cell_S;        // [nc*3] sum over the cell's faces of +-A n (0 for a closed, planar-faced cell)
+    const double *cell_S;       // [nc*3] sum over the cell's faces of +-A n (0 for a closed, planar-faced cell)
     const int *face_bnd;        // [nf] boundary index or -1
     // per-cell stencil tables, padded to MAXF faces / MAXN neighbours
     // (2D: 4, 3D: 6), in the cell's local face order
