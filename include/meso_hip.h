@@ -174,6 +174,23 @@ int meso_case_setting_double(const meso_case *c, const char *key, double dflt, d
 /* Writes at most cap bytes (NUL included). */
 int meso_case_setting_str(const meso_case *c, const char *key, const char *dflt, char *out, int32_t cap);
 
+/* The files the solver's output() writes for `step` (macros of the case's
+ * cells, as meso_hip_get_macros / meso_hip_get_flux_moments return them), under
+ * root/<case-name>/: grid.plt at step 0, step-N/data.plt (Tecplot), latest/ and,
+ * with output-np True, step-N/ text fields (Rho, U, T, q, fluxM0, fluxM1,
+ * fluxM2; BGK: Rho, U); DUGKS: result-N.dat.plt and np-data/ (Rho, vel).
+ * Formats of Field::output (src/solver/field.cpp:139-173) and Mesh::output_grid /
+ * output_data / output (src/mesh/mesh_output.cpp:111-330); layout of
+ * CDUGKS_SHAKHOV::output (cdugks_shakhov.cpp:791-855), CDUGKS::output
+ * (cdugks.cpp:373-420), DUGKS::output (dugks.cpp:385-411). T, q and the flux
+ * moments may be null except for Shakhov; Shakhov's flux moments are all three
+ * null before the first step (meso's flux fields are then empty and so are
+ * fluxM0-2). root null or "" = working directory. */
+int meso_case_write_output(const meso_case *c, const char *root, int32_t step, double solution_time,
+                           const double *rho, const double *u3, const double *T, const double *q3,
+                           const double *flux_m0, const double *flux_m1_3, const double *flux_m2);
+const char *meso_case_output_error(void);
+
 /* ======================================================================== */
 /* Device step                                                               */
 /* ======================================================================== */

@@ -87,6 +87,9 @@ def lib() -> C.CDLL:
             "meso_case_setting_int": ([vp, C.c_char_p, C.c_int32, _i], C.c_int),
             "meso_case_setting_double": ([vp, C.c_char_p, C.c_double, _d], C.c_int),
             "meso_case_setting_str": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32], C.c_int),
+            "meso_case_write_output": ([vp, C.c_char_p, C.c_int32, C.c_double, _d, _d, _d, _d, _d, _d, _d],
+                                       C.c_int),
+            "meso_case_output_error": ([], C.c_char_p),
             "meso_hip_create": ([C.POINTER(MeshDesc), C.POINTER(DvsDesc), C.POINTER(StepParams), C.c_int32,
                                  C.POINTER(vp)], C.c_int),
             "meso_hip_destroy": ([vp], None),
@@ -187,6 +190,19 @@ class Case:
         rho, u, T = np.zeros(nc), np.zeros(nc * 3), np.zeros(nc)
         self._check(lib().meso_case_initial_patch(self._h, dptr(rho), dptr(u), dptr(T)))
         return rho, u.reshape(nc, 3), T
+
+    def write_output(self, root, step, solution_time, rho, u, T=None, q=None, flux=None):
+        """The files meso's output() writes for `step` (meso_case_write_output):
+        rho (NC), u (NC x 3); Shakhov also T, q (NC x 3) and flux = (m0, m1, m2)."""
+        def arr(x, n):
+            return None if x is None else np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(n))
+        nc = self.n_cells
+        m0, m1, m2 = flux if flux is not None else (None, None, None)
+        bufs = [arr(rho, nc), arr(u, 3 * nc), arr(T, nc), arr(q, 3 * nc), arr(m0, nc), arr(m1, 3 * nc), arr(m2, nc)]
+        ptrs = [None if b is None else dptr(b) for b in bufs]
+        rc = lib().meso_case_write_output(self._h, str(root).encode(), step, solution_time, *ptrs)
+        if rc != MESO_OK:
+            raise MesoError(rc, lib().meso_case_output_error().decode())
 
     def setting_int(self, key, dflt):
         out = C.c_int32()

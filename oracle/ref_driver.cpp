@@ -1,13 +1,13 @@
 // Driver for meso itself, compiled from the reference sources where they lie
 // (see oracle/Makefile.ref). It constructs meso's own solver class from a case
 // file, runs initial() and do_step() exactly as handle_solver<T>() does
-// (src/meso/exec.cpp:141-160, minus output()), and dumps geometry and state as
-// raw little-endian float64/int32 files for golden vectors.
+// (src/meso/exec.cpp:141-160; output() only with --write), and dumps geometry
+// and state as raw little-endian float64/int32 files for golden vectors.
 //
 // TEST INFRASTRUCTURE ONLY: used to write tests/golden/ fixtures and to pin
 // the restatement in oracle/meso_oracle.cpp. Nothing in the product uses it.
 //
-// usage: meso_ref --case <case.txt> --out <dir> --dump 0,1,2,10 [--state]
+// usage: meso_ref --case <case.txt> --out <dir> --dump 0,1,2,10 [-state] [-write]
 #include "solver/solver.h"
 
 #include <cstdio>
@@ -125,8 +125,10 @@ void dump_macros(Solver::CDUGKS &s, int step, bool state) { dump_bgk(s, step, st
 template <>
 void dump_macros(Solver::DUGKS &s, int step, bool state) { dump_bgk(s, step, state); }
 
+// write: also call the solver's own output() at the dumped steps (files under
+// the working directory, as meso writes them)
 template <class S>
-int run(ArgParser &parser, Solver::Config &config, const std::set<int> &steps, bool state) {
+int run(ArgParser &parser, Solver::Config &config, const std::set<int> &steps, bool state, bool write) {
     S solver(parser, config);
     solver.initial();
     put_geometry(solver.mesh, solver.dvs_mesh);
@@ -134,11 +136,17 @@ int run(ArgParser &parser, Solver::Config &config, const std::set<int> &steps, b
         double dt = solver.dt;
         put("dt.f64", &dt, 8);
     }
-    if (steps.count(0)) dump_macros(solver, 0, state);
+    if (steps.count(0)) {
+        dump_macros(solver, 0, state);
+        if (write) solver.output();
+    }
     int last = steps.empty() ? 0 : *steps.rbegin();
     for (int i = 1; i <= last; ++i) {
         solver.do_step();
-        if (steps.count(i)) dump_macros(solver, i, state);
+        if (steps.count(i)) {
+            dump_macros(solver, i, state);
+            if (write) solver.output();
+        }
         if (!solver.get_run_state()) break;
     }
     return 0;
@@ -153,14 +161,15 @@ int main(int argc, char **argv) {
     g_out = parser.parse_param<std::string>("out", "ref_out", true);
     std::string dump = parser.parse_param<std::string>("dump", "0,1", false);
     bool state = parser.parse_switch("state");
+    bool write = parser.parse_switch("write");
     ::mkdir(g_out.c_str(), 0755);
     Solver::Config config(case_file);
     std::string solver = config.get<std::string>("solver", "<solver>", true);
     auto steps = parse_steps(dump);
     int rc = 0;
-    if (solver == "cdugks@shakhov") rc = run<Solver::CDUGKS_SHAKHOV>(parser, config, steps, state);
-    else if (solver == "cdugks@incompressible") rc = run<Solver::CDUGKS>(parser, config, steps, state);
-    else if (solver == "dugks@incompressible") rc = run<Solver::DUGKS>(parser, config, steps, state);
+    if (solver == "cdugks@shakhov") rc = run<Solver::CDUGKS_SHAKHOV>(parser, config, steps, state, write);
+    else if (solver == "cdugks@incompressible") rc = run<Solver::CDUGKS>(parser, config, steps, state, write);
+    else if (solver == "dugks@incompressible") rc = run<Solver::DUGKS>(parser, config, steps, state, write);
     else rc = 2;
     MESO::MPI::Finalize();
     return rc;

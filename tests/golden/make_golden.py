@@ -15,7 +15,11 @@ seeds, except re400_cavity and kn0075_cavity, which use meso's shipped
 files/cavity.neu (1928 mixed quad/triangle cells), files/gh-3.dvs,
 files/nc.dvs and the settings of files/case-re400.txt / files/case-kn0.075.txt.
 
-usage: python tests/golden/make_golden.py [--ref-root /root/reference]
+The out_* fixtures hold the files meso's own output() wrote (case-name/
+grid.plt, step-N/data.plt, latest/ and step-N/ text fields; DUGKS result
+files and np-data/) after steps 0 and 2, for the output writer's tests.
+
+usage: python tests/golden/make_golden.py [--ref-root /root/reference] [cases]
 """
 import argparse
 import os
@@ -111,6 +115,42 @@ CASES = {
 }
 
 
+# output fixtures: name -> (input case, steps); output-np True is added
+OUT_CASES = {
+    "out_shak_quad6": ("shak_quad6", [0, 2]),
+    "out_bgk_quad8": ("bgk_quad8", [0, 2]),
+    "out_dugks_quad8": ("dugks_quad8", [0, 2]),
+    "out_shak_hex3": ("shak_hex3", [0, 2]),
+}
+
+
+def run_ref_output(case, steps, dest):
+    """meso's output() files of `case` at `steps` into dest/."""
+    work = tempfile.mkdtemp()
+    try:
+        src = os.path.dirname(case)
+        for f in ("mesh.neu", "dvs.dvs"):
+            shutil.copy(os.path.join(src, f), work)
+        lines = [ln for ln in open(os.path.join(src, "case.txt")) if not ln.startswith("output-np")]
+        i = next(k for k, ln in enumerate(lines) if ln.startswith("[settings]"))
+        lines.insert(i + 1, "output-np            True\n")
+        with open(os.path.join(work, "case.txt"), "w") as fp:
+            fp.writelines(lines)
+        cmd = [REF_BIN, "--case", "case.txt", "--out", os.path.join(work, "raw"), "--dump",
+               ",".join(map(str, steps)), "-write"]
+        r = subprocess.run(cmd, cwd=work, capture_output=True, text=True)
+        if r.returncode:
+            raise RuntimeError(r.stdout[-2000:] + r.stderr[-2000:])
+        shutil.rmtree(dest, ignore_errors=True)
+        os.makedirs(dest)
+        for f in ("case.txt", "mesh.neu", "dvs.dvs"):
+            shutil.copy(os.path.join(work, f), dest)
+        name = next(ln.split()[1] for ln in lines if ln.split()[:1] == ["case-name"])
+        shutil.copytree(os.path.join(work, name), os.path.join(dest, "out", name))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def run_ref(case, steps, state_step):
     work = tempfile.mkdtemp()
     try:
@@ -161,7 +201,17 @@ def main():
     args = ap.parse_args()
     if not os.path.exists(REF_BIN):
         sys.exit(f"{REF_BIN} missing: run make -f oracle/Makefile.ref first")
-    for name in args.cases or CASES:
+    for name in args.cases or list(CASES) + list(OUT_CASES):
+        if name in OUT_CASES:
+            base, steps = OUT_CASES[name]
+            tmp = tempfile.mkdtemp()
+            try:
+                case = gen_inputs(base, os.path.join(tmp, base), args.ref_root)
+                run_ref_output(case, steps, os.path.join(HERE, name))
+            finally:
+                shutil.rmtree(tmp, ignore_errors=True)
+            print(f"{name}: meso output files of {base} at steps {steps}", flush=True)
+            continue
         steps, state_step = CASES[name]
         d = os.path.join(HERE, name)
         case = gen_inputs(name, d, args.ref_root)
