@@ -3,8 +3,8 @@ output every write-interval steps and at the end) on the cases whose output
 meso itself wrote (tests/golden/out_*, steps 0 and 2).
 
 The run is FP64; its files must have meso's names, layout and text, with every
-number within 1e-9 of meso's (relative to the largest magnitude in the file:
-the GPU sums in another order). grid.plt holds geometry only and must match
+number within 1e-9 of meso's (relative to the largest magnitude in the file,
+at least 1e-6: the GPU sums in another order). grid.plt holds geometry only and must match
 byte for byte. A resume (read-np-data True, --not-run) writes step 0 from the
 latest/ files it read, byte for byte.
 """
@@ -80,7 +80,9 @@ def test_runner_matches_meso_output(name, tmp_path):
         assert wa == wb, f
         assert na.shape == nb.shape, f
         if nb.size:
-            err = np.abs(na - nb).max() / max(np.abs(nb).max(), 1e-300)
+            # a field that is zero to round-off (u at step 0: |u| ~ 1e-17 from the
+            # discrete moments of the equilibrium) is measured against 1e-6
+            err = np.abs(na - nb).max() / max(np.abs(nb).max(), 1e-6)
             assert err < TOL, f"{f}: {err:.2e}"
 
     if not os.path.isdir(os.path.join(mine, "latest")):
