@@ -74,6 +74,9 @@ def case_factory(tmp_path_factory):
         elif name in ("shak_channel_pressure", "shak_channel_farfield", "shak_sym_slip", "bgk_channel_farfield",
                       "shak_sym_slip_lim"):
             p = channel_case(d, name)
+        elif name in ("neumann", "kn0075_cavity", "re400_cavity"):
+            # meso's own example cases with its shipped meshes and velocity sets
+            p = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", name, "case.txt")
         else:
             raise KeyError(name)
         cache[name] = p

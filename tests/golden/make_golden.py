@@ -13,7 +13,9 @@ sources by oracle/Makefile.ref (build it first: make -f oracle/Makefile.ref),
 run with one MPI rank. Inputs are generated by meso_hip.meshgen with fixed
 seeds, except re400_cavity and kn0075_cavity, which use meso's shipped
 files/cavity.neu (1928 mixed quad/triangle cells), files/gh-3.dvs,
-files/nc.dvs and the settings of files/case-re400.txt / files/case-kn0.075.txt.
+files/nc.dvs and the settings of files/case-re400.txt / files/case-kn0.075.txt,
+and neumann, which is meso's files/case-neumann.txt with its two meshes
+(files/neumann/: the physical mesh and the unstructured velocity mesh).
 
 The out_* fixtures hold the files meso's own output() wrote (case-name/
 grid.plt, step-N/data.plt, latest/ and step-N/ text fields; DUGKS result
@@ -93,6 +95,27 @@ def gen_inputs(name, d, ref_root):
         with open(os.path.join(d, "case.txt"), "w") as fp:
             fp.writelines(lines)
         return os.path.join(d, "case.txt")
+    if name == "neumann":
+        # meso's files/case-neumann.txt: four-state Riemann problem on
+        # files/neumann/neumann_phy.neu (quads) with the unstructured gambit DVS
+        # files/neumann/neumann_dvs.neu (2990 triangle centroids), gas-k 2
+        os.makedirs(d, exist_ok=True)
+        files = os.path.join(ref_root, "files")
+        shared = os.path.join(HERE, "neumann_mesh")
+        os.makedirs(shared, exist_ok=True)
+        for f in ("neumann_phy.neu", "neumann_dvs.neu"):
+            shutil.copy(os.path.join(files, "neumann", f), os.path.join(shared, f))
+        lines = []
+        for line in open(os.path.join(files, "case-neumann.txt")):
+            tok = line.split()
+            if tok and tok[0] == "mesh-file":
+                line = "mesh-file       ../neumann_mesh/neumann_phy.neu\n"
+            elif tok and tok[0] == "dvs-file":
+                line = "dvs-file        ../neumann_mesh/neumann_dvs.neu\n"
+            lines.append(line)
+        with open(os.path.join(d, "case.txt"), "w") as fp:
+            fp.writelines(lines)
+        return os.path.join(d, "case.txt")
     raise KeyError(name)
 
 
@@ -112,6 +135,7 @@ CASES = {
     "bgk_channel_farfield": ([1, 10], None),
     "re400_cavity": ([1, 10, 100], None),
     "kn0075_cavity": ([1, 2], None),
+    "neumann": ([1, 2, 5], None),
 }
 
 
@@ -157,8 +181,10 @@ def run_ref(case, steps, state_step):
         for f in os.listdir(os.path.dirname(case)):
             if f in ("case.txt", "mesh.neu", "dvs.dvs"):
                 shutil.copy(os.path.join(os.path.dirname(case), f), work)
-        shutil.copytree(os.path.join(HERE, "cavity_mesh"), os.path.join(os.path.dirname(work), "cavity_mesh"),
-                        dirs_exist_ok=True)
+        for shared in ("cavity_mesh", "neumann_mesh"):
+            if os.path.isdir(os.path.join(HERE, shared)):
+                shutil.copytree(os.path.join(HERE, shared), os.path.join(os.path.dirname(work), shared),
+                                dirs_exist_ok=True)
         out = os.path.join(work, "out")
         dump = sorted(set([0] + steps + ([state_step] if state_step else [])))
         cmd = [REF_BIN, "--case", "case.txt", "--out", out, "--dump", ",".join(map(str, dump))]
