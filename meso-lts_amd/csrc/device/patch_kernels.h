@@ -26,6 +26,10 @@
 #pragma once
 
 constexpr int BLOCKP = 512;  // threads of a patch block
+// Owned cells per thread slot of an update block: a patch owns at most
+// UPD_QJ * (BLOCKP / VT) cells (each slot keeps its cells' heat-flux partials
+// in registers over all tiles).
+constexpr int UPD_QJ = 2;
 
 // Per-cell constants of the update (written by k_cell_prep for interior cells).
 template <class S, int D>
@@ -524,7 +528,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
 // LDS once per block (read per lane from HBM in every tile they would cost
 // address arithmetic and L2 latency).
 template <int MODEL, class S, int VT, bool TAB, int D, bool LIM>
-__global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
+__global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.halt) return;
     constexpr bool SH = MODEL == CDUGKS_SHAKHOV;
@@ -544,18 +548,17 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     const int r0 = a.up_row_ptr[p], nr = a.up_row_ptr[p + 1] - r0;
     const int nown = a.up_nown[p];
     const int maxi = a.up_maxi, maxf = a.up_maxf, maxr = a.up_maxr;
-    // LDS: items | faces | rows | values | face fluxes | q partials
+    // LDS: items | faces | rows | values | face fluxes | face and cell records
     auto *items = reinterpret_cast<Item *>(lds_raw);
     auto *faces = reinterpret_cast<Face *>(items + maxi);
     int *rows = reinterpret_cast<int *>(faces + maxf);
     S *val = reinterpret_cast<S *>(lds_raw + ((reinterpret_cast<unsigned char *>(rows + maxr) - lds_raw + 15) & ~15));
     S *flx = val + size_t(ND) * maxr * VT;
-    double *qacc = reinterpret_cast<double *>(
-        lds_raw + ((reinterpret_cast<unsigned char *>(flx + size_t(ND) * maxf * VT) - lds_raw + 15) & ~15));
     // the face equilibria and mix weights of the patch faces and the update
     // constants of the owned cells, copied once per block
     using EqS = typename UpdateRecs<S>::Eq;
-    EqS *feq = reinterpret_cast<EqS *>(qacc + D * a.up_maxo);
+    EqS *feq = reinterpret_cast<EqS *>(
+        lds_raw + ((reinterpret_cast<unsigned char *>(flx + size_t(ND) * maxf * VT) - lds_raw + 15) & ~15));
     double *fmix = reinterpret_cast<double *>(feq + maxf);
     CellRec<S, D> *crl = reinterpret_cast<CellRec<S, D> *>(fmix + maxf);
     CellS<D> *csl = reinterpret_cast<CellS<D> *>(crl + a.up_maxo);
@@ -569,7 +572,6 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     stage_copy(up_items + i0, ni, items);
     stage_copy(up_faces + f0, nfp, faces);
     for (int r = threadIdx.x; r < nr; r += BLOCKP) rows[r] = a.up_rows[r0 + r];
-    for (int r = threadIdx.x; r < D * nown; r += BLOCKP) qacc[r] = 0.0;
     for (int lf = threadIdx.x; lf < nfp; lf += BLOCKP) {
         const int f = up_faces[f0 + lf].face;
         feq[lf] = UpdateRecs<S>::face_eq(a, f);
@@ -582,6 +584,12 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
     }
     const int ntiles = (a.s.nv + VT - 1) / VT;
     const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
+    // Shakhov heat-flux partials of this thread's owned cells over the chunk
+    double qreg[UPD_QJ][D];
+#pragma unroll
+    for (int j = 0; j < UPD_QJ; ++j)
+#pragma unroll
+        for (int i = 0; i < D; ++i) qreg[j][i] = 0.0;
     __syncthreads();
     for (int t = t0; t < t1; ++t) {
         const int v = t * VT + vl;
@@ -590,7 +598,9 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
         // A: stage values
         stage_values<ND, S, VT>(rows, nr, fin_g, fin_h, nvp, v, L.ok, slot, vl, val, maxr);
         __syncthreads();
-        // B: gradients; flux (xi.n) A f_face of each face on its upwind side's lanes
+        // B1: gradients; each face's half-step value f-bar+ on its upwind side's
+        // lanes, written to the face's flux slot (interior dead-band lanes write
+        // nothing: their flux is 0)
         for (int s = slot; s < ni; s += NSLOT) {
             const Item *it = items + s;
             const int nface = it->nface;
@@ -613,51 +623,61 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
                 double sn = 0.0;
 #pragma unroll
                 for (int i = 0; i < D; ++i) sn += F->n[i] * xv[i];
-                const bool bnd = mode == meso_patch::SIDE_BOUNDARY;
-                const bool own = mode == meso_patch::SIDE_OWNER;
-                const bool dead = !(sn >= VSMALL) && !(sn < -VSMALL);
-                const bool up = bnd || (own ? sn >= VSMALL : sn < -VSMALL);
-                if (!__any(up || (own && dead))) continue;
-                if (up) {
-                    const int f = F->face;
-                    double gs, hs = 0.0;
-                    face_eq_rec<MODEL, S, TAB, D>(a, feq[lf], f, L, gs, hs);
-                    const double mix = fmix[lf];
-                    const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
-                    double ev[D];
+                const bool up = mode == meso_patch::SIDE_BOUNDARY ||
+                                (mode == meso_patch::SIDE_OWNER ? sn >= VSMALL : sn < -VSMALL);
+                const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
+                double ev[D];
 #pragma unroll
-                    for (int i = 0; i < D; ++i) ev[i] = F->d[side][i] - xv[i] * hdt;
-                    const double knA = sn * F->area;
-                    double fm[2] = {0.0, 0.0};
+                for (int i = 0; i < D; ++i) ev[i] = F->d[side][i] - xv[i] * hdt;
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) {
-                        double Xd = 0.0;
+                for (int d = 0; d < ND; ++d) {
+                    double Xd = 0.0;
 #pragma unroll
-                        for (int i = 0; i < D; ++i) Xd += ev[i] * g[d][i];
-                        if constexpr (LIM) {
-                            double dw = 0.0;
+                    for (int i = 0; i < D; ++i) Xd += ev[i] * g[d][i];
+                    if constexpr (LIM) {
+                        double dw = 0.0;
 #pragma unroll
-                            for (int i = 0; i < D; ++i) dw += F->d[side][i] * g[d][i];
-                            Xd *= venkata_phi(dw, fc[d], mn[d], mx[d], omega, sn >= VSMALL ? 1.0e-10 : VSMALL);
-                        }
-                        // a boundary face's dead-band value 0.5 (f + f + X + X) is f + X
-                        const double fbar = fc[d] + Xd;
-                        const double eq = d == 0 ? gs : hs;
-                        if constexpr (sizeof(S) == 8) fm[d] = (1.0 - mix) * fbar + mix * eq;
-                        else fm[d] = fbar + mix * (eq - fbar);
+                        for (int i = 0; i < D; ++i) dw += F->d[side][i] * g[d][i];
+                        Xd *= venkata_phi(dw, fc[d], mn[d], mx[d], omega, sn >= VSMALL ? 1.0e-10 : VSMALL);
                     }
-                    if (bnd) bnd_value_lane<MODEL, S, D>(a, F->face, sn, L, fm[0], fm[1], fm[0], fm[1]);
-#pragma unroll
-                    for (int d = 0; d < ND; ++d) flx[(d * maxf + lf) * VT + vl] = S(knA * fm[d]);
-                } else if (own && dead) {
-#pragma unroll
-                    for (int d = 0; d < ND; ++d) flx[(d * maxf + lf) * VT + vl] = S(0);
+                    // a boundary face's dead-band value 0.5 (f + f + X + X) is f + X
+                    if (up) flx[(d * maxf + lf) * VT + vl] = S(fc[d] + Xd);
                 }
             }
         }
         __syncthreads();
-        // C: owned cells
-        for (int s = slot; s < nown; s += NSLOT) {
+        // B2: per face and velocity (all lanes busy): equilibrium mix of f-bar+,
+        // the boundary value, flux (xi.n) A f_face into the same slot
+        if (L.ok)
+            for (int lf = threadIdx.x / VT; lf < nfp; lf += BLOCKP / VT) {
+                const Face *F = faces + lf;
+                double sn = 0.0;
+#pragma unroll
+                for (int i = 0; i < D; ++i) sn += F->n[i] * xv[i];
+                const bool bnd = F->own == F->nei;
+                const bool dead = !(sn >= VSMALL) && !(sn < -VSMALL);
+                const int f = F->face;
+                double gs, hs = 0.0;
+                face_eq_rec<MODEL, S, TAB, D>(a, feq[lf], f, L, gs, hs);
+                const double mix = fmix[lf];
+                double fm[2] = {0.0, 0.0};
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const double fbar = double(flx[(d * maxf + lf) * VT + vl]);
+                    const double eq = d == 0 ? gs : hs;
+                    if constexpr (sizeof(S) == 8) fm[d] = (1.0 - mix) * fbar + mix * eq;
+                    else fm[d] = fbar + mix * (eq - fbar);
+                }
+                if (bnd) bnd_value_lane<MODEL, S, D>(a, f, sn, L, fm[0], fm[1], fm[0], fm[1]);
+                const double knA = sn * F->area;
+#pragma unroll
+                for (int d = 0; d < ND; ++d)
+                    flx[(d * maxf + lf) * VT + vl] = S(dead && !bnd ? 0.0 : knA * fm[d]);
+            }
+        __syncthreads();
+        // C: owned cells (this thread's are slot + j NSLOT, the same in every tile)
+#pragma nounroll
+        for (int s = slot, j = 0; s < nown; s += NSLOT, ++j) {
             const Item *it = items + s;
             const int c = it->cell;
             const int nface = it->nface;
@@ -765,23 +785,31 @@ __global__ __launch_bounds__(BLOCKP) void k_patch_update(KArgs a) {
 #pragma unroll
                 for (int i = 0; i < D; ++i) qv[i] = 0.0;
             }
-            if (SH) {
-                // sum over the VT lanes of this item
+            if (SH)
 #pragma unroll
-                for (int o = VT / 2; o > 0; o >>= 1)
+                for (int jj = 0; jj < UPD_QJ; ++jj)
+                    if (jj == j)
 #pragma unroll
-                    for (int i = 0; i < D; ++i) qv[i] += __shfl_xor(qv[i], o, 64);
-                if (vl == 0)
-#pragma unroll
-                    for (int i = 0; i < D; ++i) qacc[D * s + i] += qv[i];
-            }
+                        for (int i = 0; i < D; ++i) qreg[jj][i] += qv[i];
         }
         __syncthreads();
     }
     if (SH)
-        for (int r = threadIdx.x; r < nown; r += BLOCKP) {
-            const int c = items[r].cell;
-            double *dst = a.part_c + (size_t(c) * a.s.ntile_c + chunk) * 4;
-            for (int i = 0; i < 4; ++i) dst[i] = i < D ? qacc[D * r + i] : 0.0;
+#pragma unroll
+        for (int j = 0; j < UPD_QJ; ++j) {
+            // sum over the VT lanes of each owned cell (all lanes take part)
+            const int s = slot + j * NSLOT;
+            double qv[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) qv[i] = qreg[j][i];
+#pragma unroll
+            for (int o = VT / 2; o > 0; o >>= 1)
+#pragma unroll
+                for (int i = 0; i < D; ++i) qv[i] += __shfl_xor(qv[i], o, 64);
+            if (s < nown && vl == 0) {
+                double *dst = a.part_c + (size_t(items[s].cell) * a.s.ntile_c + chunk) * 4;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) dst[i] = i < D ? qv[i] : 0.0;
+            }
         }
 }

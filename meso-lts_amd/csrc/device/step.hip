@@ -2104,12 +2104,13 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     // cells x 16^3 velocities); FP32 (32-velocity tiles) gains nothing from it
     const size_t cap_up = D == 3 && storage == MESO_STORAGE_FP64 ? size_t(lds_blk) : cap;
     bool ok = true;
-    for (int P = env_int("MESO_HIP_PATCH_CELLS", 32); ok; P = P * 3 / 4) {
+    // owned cells per update patch: at most UPD_QJ per thread slot
+    const int pmax = UPD_QJ * NWP * IPWU;
+    for (int P = std::min(env_int("MESO_HIP_PATCH_CELLS", 32), pmax); ok; P = P * 3 / 4) {
         ok = meso_patch::build_update_patches<D>(mv, pown, P, ups, perr);
-        // items | faces | rows | values | face fluxes | q partials | face and cell records (k_patch_update)
+        // items | faces | rows | values | face fluxes | face and cell records (k_patch_update)
         c->lds_up = rec(IPWU, ups.max_items, ups.max_faces) + r16(4 * size_t(ups.max_rows)) +
                     r16(size_t(ND) * ups.max_rows * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
-                    size_t(ups.max_owned) * 8 * D +
                     (storage == MESO_STORAGE_FP32 ? update_rec_bytes<float, D>(ups.max_faces, ups.max_owned)
                                                   : update_rec_bytes<double, D>(ups.max_faces, ups.max_owned));
         if (c->lds_up <= cap_up || P <= 4) break;
@@ -2125,7 +2126,9 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
         c->lds_mp = std::max(a1, a2);
         if (c->lds_mp <= cap || F <= 4) break;
     }
-    if (!ok || c->lds_up > cap_up || c->lds_mp > cap || ups.n_patch == 0 || mps.n_patch == 0) return false;
+    if (!ok || c->lds_up > cap_up || c->lds_mp > cap || ups.n_patch == 0 || mps.n_patch == 0 ||
+        ups.max_owned > pmax)
+        return false;
     c->patches = true;
     c->p_vchunk = std::max(64, (env_int("MESO_HIP_PCHUNK", 2560) + 63) / 64 * 64);
     c->p_chunks = (nv + c->p_vchunk - 1) / c->p_vchunk;
