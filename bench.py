@@ -15,6 +15,7 @@ OpenMP on this host, one step of the same workload).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scale s]
                        [--storage both|fp32|fp64] [--copy-calibration MIB]
+                       [--batch N [--autotune]]
 """
 import argparse
 import json
@@ -94,6 +95,58 @@ def copy_calibration(mib):
     log(f"[bench] copy calibration: 3 x {mib} MiB copies done")
 
 
+def batch_main(args):
+    """The batched cavity sweep: N independent Shakhov cavities (quads 32^2 to
+    128^2, Kn log-uniform in [0.01, 10], lid speed U[0.05, 0.3], Newton-Cotes
+    16^2 to 64^2 velocities, seeds 1000 + i), FP32 storage, advanced together
+    with meso_hip_step_batch; value = sum of NC * NV * steps / seconds. Also
+    timed: the same instances stepped one after the other (meso_hip_step)."""
+    work = os.path.join(tempfile.gettempdir(), "meso_hip_bench")
+    insts = configs.batch_instances(args.batch)
+    cases = [capi.Case(configs.build_instance(i, work, args.scale)) for i in insts]
+    devs = []
+    chunks = []
+    for cs in cases:
+        d = capi.Device(cs, capi.STORAGE_FP32)
+        d.init_equilibrium()
+        chunks.append(d.autotune(3) if args.autotune else 0)
+        devs.append(d)
+    dvc = sum(cs.n_cells * cs.n_vel for cs in cases)
+    log(f"[bench] batch of {len(cases)}: {dvc:.4e} DV-cells in total "
+        f"(cells {min(c.n_cells for c in cases)}..{max(c.n_cells for c in cases)}, "
+        f"velocities {min(c.n_vel for c in cases)}..{max(c.n_vel for c in cases)})")
+    capi.step_batch(devs, args.warmup)
+    for d in devs:
+        d.sync()
+    t0 = time.perf_counter()
+    reps = capi.step_batch(devs, args.steps)
+    for d in devs:
+        d.sync()
+    secs = time.perf_counter() - t0
+    if any(r.bad_cells for r in reps):
+        raise RuntimeError("numerical failure in the batch")
+    t0 = time.perf_counter()
+    for d in devs:
+        d.step(args.steps)
+    seq = time.perf_counter() - t0
+    ups = dvc * args.steps / secs
+    log(f"[bench] batch: {secs / args.steps * 1e3:.3f} ms per step of all instances, {ups:.4e} DV-cell updates/s; "
+        f"one after the other: {seq / args.steps * 1e3:.3f} ms")
+    line = {
+        "metric": METRIC, "value": ups, "unit": "DV-cell updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch_{len(cases)}", "instances": len(cases), "dv_cells": dvc,
+                   "autotune": bool(args.autotune), "storage": "fp32 g,h; fp64 moments/macros"},
+        "sequential_ms_per_step": seq / args.steps * 1e3,
+        "batch_over_sequential": seq / secs,
+        "vchunks": chunks if args.autotune else None,
+    }
+    for d in devs:
+        d.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,9 +160,15 @@ def main():
                     help="storage kinds to run (profiling runs use one; the JSON line needs both)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
+    ap.add_argument("--batch", type=int, default=0, metavar="N",
+                    help="instead: the batched cavity sweep of N seeded instances (FP32 storage)")
+    ap.add_argument("--autotune", action="store_true", help="with --batch: autotune each instance first")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.batch:
+        batch_main(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the step is benchmarked on one MI355X (--gpus {args.gpus} "

@@ -221,6 +221,22 @@ int meso_hip_get_state(meso_hip_ctx *ctx, double *f, int32_t layout);
 int meso_hip_step(meso_hip_ctx *ctx, int32_t nsteps, meso_step_report *report);
 int meso_hip_set_params(meso_hip_ctx *ctx, const meso_step_params *params);
 
+/* Many independent cases at once (the batched cavity sweep): advances each of
+ * the n contexts by nsteps, issuing step i of every context (each on its own
+ * stream) before step i + 1 of any and waiting once per context at the end, so
+ * that small cases share the GPU. reports[k] (may be NULL) is what
+ * meso_hip_step would report for ctxs[k]; returns the first non-OK status. A
+ * context may appear once per call. No counterpart in the reference, which runs
+ * one case per process (src/meso/exec.cpp:141-165). */
+int meso_hip_step_batch(meso_hip_ctx *const *ctxs, int32_t n, int32_t nsteps, meso_step_report *reports);
+/* Autotuner of the cell x velocity tile of the patch kernels: times
+ * trial_steps steps of the current state (computed but not committed: the
+ * state does not advance) for velocity chunks of half, the same and twice the
+ * chunk count, keeps the fastest and returns its velocities per block
+ * (0 where the per-cell kernels run). The default chunk already gives 16
+ * blocks per CU where the velocity set allows. No counterpart in the reference. */
+int meso_hip_autotune(meso_hip_ctx *ctx, int32_t trial_steps, int32_t *vchunk);
+
 /* Cell macros after the last step: rho[NC], u3[NC*3], T[NC], q3[NC*3] (NULL skips). */
 int meso_hip_get_macros(meso_hip_ctx *ctx, double *rho, double *u3, double *T, double *q3);
 /* Cell flux moments of the last step (fluxM0..2 of cdugks_shakhov.cpp:837-839). */

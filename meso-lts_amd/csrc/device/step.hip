@@ -1558,6 +1558,7 @@ struct meso_hip_ctx {
     int n_up = 0, up_maxi = 0, up_maxf = 0, up_maxo = 0, up_maxr = 0, n_mp = 0, mp_maxi = 0, mp_maxf = 0,
         mp_maxr = 0;
     int p_chunks = 0, p_vchunk = 0;
+    int p_chunk_cap = 0;        // chunks the part_f / part_c buffers hold (>= p_chunks)
     size_t lds_up = 0, lds_mp = 0;
     bool upd_attr[2] = {false, false};  // the dynamic-LDS limit of k_patch_update<.., LIM> has been raised
     bool mp_attr[2] = {false, false};   // the same for k_patch_moments
@@ -2059,6 +2060,24 @@ int copy_residual_base(meso_hip_ctx *c) {
     return MESO_OK;
 }
 
+// Velocities per block of the patch kernels (a multiple of 64). At most 2560:
+// the records a block stages once are amortised over its chunk's tiles; and
+// enough chunks that the smaller patch grid has 16 blocks per CU, so that small
+// cases fill the GPU (as far as 64-velocity chunks allow).
+int default_vchunk(int nv, int npatch, int ncu) {
+    const int most = (nv + 63) / 64;
+    int chunks = std::max((nv + 2559) / 2560, (16 * ncu + npatch - 1) / std::max(npatch, 1));
+    chunks = std::max(1, std::min(chunks, most));
+    return ((nv + chunks - 1) / chunks + 63) / 64 * 64;
+}
+
+void set_vchunk(meso_hip_ctx *c, int vchunk) {
+    c->p_vchunk = vchunk;
+    c->p_chunks = (c->s.nv + vchunk - 1) / vchunk;
+    c->s.ntile_c = c->s.ntile_f = c->p_chunks;
+    c->s.vt_c = c->s.vt_f = c->p_vchunk;
+}
+
 // Spatial patches for the LDS-staged kernels: builds the update and moment
 // patch sets, shrinking the patches until each block fits its LDS budget
 // (half the LDS of a CU, so that two blocks share a CU). Sets the patch fields
@@ -2130,10 +2149,13 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
         ups.max_owned > pmax)
         return false;
     c->patches = true;
-    c->p_vchunk = std::max(64, (env_int("MESO_HIP_PCHUNK", 2560) + 63) / 64 * 64);
-    c->p_chunks = (nv + c->p_vchunk - 1) / c->p_vchunk;
-    c->s.ntile_c = c->s.ntile_f = c->p_chunks;
-    c->s.vt_c = c->s.vt_f = c->p_vchunk;
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+    const int env_chunk = env_int("MESO_HIP_PCHUNK", 0);
+    set_vchunk(c, env_chunk > 0 ? std::max(64, (env_chunk + 63) / 64 * 64)
+                                : default_vchunk(nv, std::min(ups.n_patch, mps.n_patch), ncu));
+    // room for twice as many chunks (meso_hip_autotune tries them)
+    c->p_chunk_cap = std::min((nv + 63) / 64, 2 * c->p_chunks);
     c->n_up = ups.n_patch;
     c->up_maxi = ups.max_items;
     c->up_maxf = ups.max_faces;
@@ -2621,9 +2643,9 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     const int ncol = c->nd == 2 ? (D == 2 ? 11 : 17) : (D == 2 ? 6 : 10);
     if ((rc = dalloc(c, &c->rho_res, nc)) || (rc = dalloc(c, &c->u_res, 3 * size_t(nc))) ||
         (rc = dalloc(c, &c->T_res, nc)) || (rc = dalloc(c, &c->q_res, 3 * size_t(nc))) ||
-        (rc = dalloc(c, &c->part_f, size_t(nf) * c->s.ntile_f * ncol)) ||
+        (rc = dalloc(c, &c->part_f, size_t(nf) * std::max(c->s.ntile_f, c->p_chunk_cap) * ncol)) ||
         (rc = dalloc(c, &c->face_flux, size_t(nf) * (D + 2))) ||
-        (rc = dalloc(c, &c->part_c, size_t(nc) * c->s.ntile_c * 4)) || (rc = dalloc(c, &c->fm, 5 * size_t(nc))) ||
+        (rc = dalloc(c, &c->part_c, size_t(nc) * std::max(c->s.ntile_c, c->p_chunk_cap) * 4)) || (rc = dalloc(c, &c->fm, 5 * size_t(nc))) ||
         (rc = dalloc(c, &c->res_part, 8 * size_t((nc + BLOCK - 1) / BLOCK))) || (rc = dalloc(c, &c->res_out, 8)) ||
         (rc = dalloc(c, &c->face_eq, nf)) || (rc = dalloc(c, &c->face_eqf, nf)) || (rc = dalloc(c, &c->bnd, std::max(1, c->s.nb))) ||
         (rc = dalloc(c, &c->flags, 4)) || (rc = dalloc(c, &c->fref_d, 2 * size_t(nv))) ||
@@ -2827,15 +2849,18 @@ int meso_hip_get_state(meso_hip_ctx *c, double *f, int32_t layout) {
     return MESO_OK;
 }
 
-int meso_hip_step(meso_hip_ctx *c, int32_t nsteps, meso_step_report *rep) {
-    if (!c || nsteps < 0) return set_err(c, MESO_E_INVALID, "bad argument");
+// Launches step `i` of a run that started at steps_done (no synchronisation).
+static int step_issue(meso_hip_ctx *c, int i) {
+    KArgs a = make_args(c);
+    a.step_index = c->steps_done + i;
+    const int rc = step_dispatch(c, a);
+    if (rc == MESO_OK) c->cur = 1 - c->cur;
+    return rc;
+}
+
+// Waits for the issued steps of a run of nsteps and reports them.
+static int step_finish(meso_hip_ctx *c, int32_t nsteps, meso_step_report *rep) {
     int rc;
-    for (int i = 0; i < nsteps; ++i) {
-        KArgs a = make_args(c);
-        a.step_index = c->steps_done + i;
-        if ((rc = step_dispatch(c, a))) return rc;
-        c->cur = 1 - c->cur;
-    }
     if ((rc = sync_flags(c))) return rc;
     const int bad = c->flags_host[0];
     int done = nsteps;
@@ -2855,6 +2880,90 @@ int meso_hip_step(meso_hip_ctx *c, int32_t nsteps, meso_step_report *rep) {
         rep->reserved = 0;
     }
     return bad > 0 ? set_err(c, MESO_E_NUMERIC, "NaN density or T<0 in a cell") : MESO_OK;
+}
+
+int meso_hip_step(meso_hip_ctx *c, int32_t nsteps, meso_step_report *rep) {
+    if (!c || nsteps < 0) return set_err(c, MESO_E_INVALID, "bad argument");
+    int rc;
+    for (int i = 0; i < nsteps; ++i)
+        if ((rc = step_issue(c, i))) return rc;
+    return step_finish(c, nsteps, rep);
+}
+
+int meso_hip_step_batch(meso_hip_ctx *const *ctxs, int32_t n, int32_t nsteps, meso_step_report *reps) {
+    if (!ctxs || n < 0 || nsteps < 0) return set_err(nullptr, MESO_E_INVALID, "bad argument");
+    for (int k = 0; k < n; ++k)
+        if (!ctxs[k]) return set_err(nullptr, MESO_E_INVALID, "null context in the batch");
+    // round-robin issue: step i of every context before step i + 1 of any, each
+    // on its context's stream; then one wait per context
+    std::vector<int> rc(n, MESO_OK), issued(n, 0);
+    for (int i = 0; i < nsteps; ++i)
+        for (int k = 0; k < n; ++k)
+            if (rc[k] == MESO_OK) {
+                rc[k] = step_issue(ctxs[k], i);
+                if (rc[k] == MESO_OK) ++issued[k];
+            }
+    int first = MESO_OK;
+    for (int k = 0; k < n; ++k) {
+        meso_step_report r{};
+        int f = step_finish(ctxs[k], issued[k], &r);
+        if (rc[k] != MESO_OK) f = rc[k];
+        if (reps) reps[k] = r;
+        if (first == MESO_OK && f != MESO_OK) {
+            first = f;
+            set_err(nullptr, f, ctxs[k]->err);
+        }
+    }
+    return first;
+}
+
+int meso_hip_autotune(meso_hip_ctx *c, int32_t trial_steps, int32_t *vchunk_out) {
+    if (!c || trial_steps < 1) return set_err(c, MESO_E_INVALID, "bad argument");
+    if (!c->patches) {  // the per-cell kernels have no velocity chunk to choose
+        if (vchunk_out) *vchunk_out = 0;
+        return MESO_OK;
+    }
+    const int nv = c->s.nv, base = c->p_chunks;
+    std::vector<int> cand;
+    for (int k : {base / 2, base, 2 * base}) {
+        if (k < 1 || k > c->p_chunk_cap) continue;
+        const int vc = ((nv + k - 1) / k + 63) / 64 * 64;
+        if ((nv + vc - 1) / vc <= c->p_chunk_cap && std::find(cand.begin(), cand.end(), vc) == cand.end())
+            cand.push_back(vc);
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(c, hipEventCreate(&e0));
+    HIP_TRY(c, hipEventCreate(&e1));
+    const int keep = c->p_vchunk;
+    int best = keep, rc = MESO_OK;
+    float best_ms = 0.0f;
+    // trial steps read the current state and write the other buffer without
+    // making it current, so the state is not advanced
+    for (int vc : cand) {
+        set_vchunk(c, vc);
+        KArgs a = make_args(c);
+        a.step_index = c->steps_done;
+        if ((rc = step_dispatch(c, a))) break;  // warm-up
+        HIP_TRY(c, hipEventRecord(e0, c->stream));
+        for (int i = 0; i < trial_steps && rc == MESO_OK; ++i) rc = step_dispatch(c, a);
+        if (rc) break;
+        HIP_TRY(c, hipEventRecord(e1, c->stream));
+        HIP_TRY(c, hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
+        if (best_ms == 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = vc;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_vchunk(c, rc == MESO_OK ? best : keep);
+    if (rc) return rc;
+    if ((rc = sync_flags(c))) return rc;
+    if (c->flags_host[0] > 0) return set_err(c, MESO_E_NUMERIC, "NaN density or T<0 in a trial step");
+    if (vchunk_out) *vchunk_out = c->p_vchunk;
+    return MESO_OK;
 }
 
 int meso_hip_get_macros(meso_hip_ctx *c, double *rho, double *u3, double *T, double *q3) {

@@ -99,6 +99,8 @@ def lib() -> C.CDLL:
             "meso_hip_set_state": ([vp, _d, C.c_int32, _d, _d, _d, _d], C.c_int),
             "meso_hip_get_state": ([vp, _d, C.c_int32], C.c_int),
             "meso_hip_step": ([vp, C.c_int32, C.POINTER(StepReport)], C.c_int),
+            "meso_hip_step_batch": ([C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(StepReport)], C.c_int),
+            "meso_hip_autotune": ([vp, C.c_int32, _i], C.c_int),
             "meso_hip_set_params": ([vp, C.POINTER(StepParams)], C.c_int),
             "meso_hip_get_macros": ([vp, _d, _d, _d, _d], C.c_int),
             "meso_hip_get_flux_moments": ([vp, _d, _d, _d], C.c_int),
@@ -298,6 +300,13 @@ class Device:
             self._check(rc)
         return rep
 
+    def autotune(self, trial_steps=3):
+        """Pick the velocity chunk of the patch kernels by timing trial steps
+        (meso_hip_autotune); returns velocities per block (0: per-cell kernels)."""
+        out = C.c_int32()
+        self._check(lib().meso_hip_autotune(self._h, trial_steps, C.byref(out)))
+        return out.value
+
     def set_params(self, p: StepParams):
         self._check(lib().meso_hip_set_params(self._h, C.byref(p)))
 
@@ -335,3 +344,16 @@ class Device:
 
     def sync(self):
         self._check(lib().meso_hip_sync(self._h))
+
+
+def step_batch(devices, n=1):
+    """Advance every Device of the list by n steps at once (meso_hip_step_batch);
+    returns the per-device StepReports (a numerical failure shows in its report,
+    as with Device.step). Raises on any other error."""
+    k = len(devices)
+    arr = (C.c_void_p * k)(*[d._h for d in devices])
+    reps = (StepReport * k)()
+    rc = lib().meso_hip_step_batch(arr, k, n, reps)
+    if rc not in (MESO_OK, MESO_E_NUMERIC):
+        raise MesoError(rc, lib().meso_hip_last_error(None).decode())
+    return list(reps)

@@ -55,6 +55,11 @@ def test_null_arguments_are_rejected():
     assert L.meso_hip_residual(None, None) == capi.MESO_E_INVALID
     assert L.meso_hip_set_params(None, None) == capi.MESO_E_INVALID
     assert L.meso_hip_device_bytes(None) == 0
+    assert L.meso_hip_autotune(None, 1, None) == capi.MESO_E_INVALID
+    assert L.meso_hip_step_batch(None, 1, 1, None) == capi.MESO_E_INVALID
+    arr = (C.c_void_p * 2)(None, None)
+    assert L.meso_hip_step_batch(arr, 2, 1, None) == capi.MESO_E_INVALID  # a null context
+    assert L.meso_hip_step_batch(arr, -1, 1, None) == capi.MESO_E_INVALID
     L.meso_hip_destroy(None)  # no-op
     h = C.c_void_p()
     assert L.meso_case_open(None, 0.0, C.byref(h)) == capi.MESO_E_INVALID
