@@ -15,7 +15,7 @@ OpenMP on this host, one step of the same workload).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scale s]
                        [--storage both|fp32|fp64] [--copy-calibration MIB]
-                       [--batch N [--autotune]]
+                       [--batch N [--autotune] [--graphs]]
 """
 import argparse
 import json
@@ -110,6 +110,7 @@ def batch_main(args):
         d = capi.Device(cs, capi.STORAGE_FP32)
         d.init_equilibrium()
         chunks.append(d.autotune(3) if args.autotune else 0)
+        d.set_graphs(args.graphs)
         devs.append(d)
     dvc = sum(cs.n_cells * cs.n_vel for cs in cases)
     log(f"[bench] batch of {len(cases)}: {dvc:.4e} DV-cells in total "
@@ -137,7 +138,8 @@ def batch_main(args):
         "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"batch_{len(cases)}", "instances": len(cases), "dv_cells": dvc,
-                   "autotune": bool(args.autotune), "storage": "fp32 g,h; fp64 moments/macros"},
+                   "autotune": bool(args.autotune), "graphs": bool(args.graphs),
+                   "storage": "fp32 g,h; fp64 moments/macros"},
         "sequential_ms_per_step": seq / args.steps * 1e3,
         "batch_over_sequential": seq / secs,
         "vchunks": chunks if args.autotune else None,
@@ -163,6 +165,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="instead: the batched cavity sweep of N seeded instances (FP32 storage)")
     ap.add_argument("--autotune", action="store_true", help="with --batch: autotune each instance first")
+    ap.add_argument("--graphs", action="store_true", help="with --batch: replay captured step pairs")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -236,6 +236,12 @@ int meso_hip_step_batch(meso_hip_ctx *const *ctxs, int32_t n, int32_t nsteps, me
  * (0 where the per-cell kernels run). The default chunk already gives 16
  * blocks per CU where the velocity set allows. No counterpart in the reference. */
 int meso_hip_autotune(meso_hip_ctx *ctx, int32_t trial_steps, int32_t *vchunk);
+/* enable != 0: meso_hip_step and meso_hip_step_batch run steps in pairs as
+ * replays of a hipGraph captured on the context's stream (one per buffer
+ * parity, captured again after any change of parameters, state or tiles;
+ * never while kernel profiling is on). Results are those of the eager steps.
+ * No counterpart in the reference. */
+int meso_hip_set_graphs(meso_hip_ctx *ctx, int32_t enable);
 
 /* Cell macros after the last step: rho[NC], u3[NC*3], T[NC], q3[NC*3] (NULL skips). */
 int meso_hip_get_macros(meso_hip_ctx *ctx, double *rho, double *u3, double *T, double *q3);

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
         if (!ok) {
             atomicAdd(a.bad_count, 1);
             atomicMin(a.first_bad, c);
-            atomicMin(a.bad_step, a.step_index);
+            atomicMin(a.bad_step, *a.step_ctr);
         }
         CellRec<S, D> r;
         if constexpr (sizeof(S) == 4) {

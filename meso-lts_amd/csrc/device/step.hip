@@ -1016,7 +1016,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
             if (!ok) {
                 atomicAdd(a.bad_count, 1);
                 atomicMin(a.first_bad, c);
-                atomicMin(a.bad_step, a.step_index);
+                atomicMin(a.bad_step, *a.step_ctr);
             }
         }
         nfc = a.cs_nf[c];
@@ -1325,7 +1325,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_finish(KArgs a) {
         if (isnan(s[0])) {
             atomicAdd(a.bad_count, 1);
             atomicMin(a.first_bad, c);
-            atomicMin(a.bad_step, a.step_index);
+            atomicMin(a.bad_step, *a.step_ctr);
         }
     }
 }
@@ -1503,12 +1503,18 @@ __global__ void k_residual_final(const double *part, int nblk, double *out) {
     }
 }
 
-__global__ void k_reset_flags(int *bad_count, int *first_bad, int *bad_step, int *halt) {
+__global__ void k_reset_flags(int *bad_count, int *first_bad, int *bad_step, int *halt, int *step_ctr, int steps) {
     *bad_count = 0;
     *first_bad = 0x7fffffff;
     *bad_step = 0x7fffffff;
     *halt = 0;
+    *step_ctr = steps;
 }
+
+// Last launch of every step: the device-side step index (read by the kernels
+// that record the first bad step; a captured step pair then needs no host
+// value that changes between replays).
+__global__ void k_step_end(int *step_ctr) { *step_ctr += 1; }
 
 #include "patch_kernels.h"
 
@@ -1559,6 +1565,13 @@ struct meso_hip_ctx {
         mp_maxr = 0;
     int p_chunks = 0, p_vchunk = 0;
     int p_chunk_cap = 0;        // chunks the part_f / part_c buffers hold (>= p_chunks)
+    // captured step pairs (meso_hip_set_graphs): gexec[p] runs two steps from
+    // buffer parity p; valid while gver[p] == kver, which changes with every
+    // change of the kernel arguments
+    bool graphs = false;
+    int kver = 0;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    int gver[2] = {-1, -1};
     size_t lds_up = 0, lds_mp = 0;
     bool upd_attr[2] = {false, false};  // the dynamic-LDS limit of k_patch_update<.., LIM> has been raised
     bool mp_attr[2] = {false, false};   // the same for k_patch_moments
@@ -1738,6 +1751,7 @@ KArgs make_args(meso_hip_ctx *c) {
     a.first_bad = c->flags + 1;
     a.bad_step = c->flags + 2;
     a.halt = c->flags + 3;
+    a.step_ctr = c->flags + 4;
     a.n_up = c->n_up;
     a.up_maxi = c->up_maxi;
     a.up_maxf = c->up_maxf;
@@ -1835,6 +1849,7 @@ int set_reference(meso_hip_ctx *c, double rho_ref, double T_ref) {
 }
 
 int set_reference_from(meso_hip_ctx *c, const double *rho, const double *T) {
+    ++c->kver;  // the reference state is a kernel argument
     if (c->storage != MESO_STORAGE_FP32) return MESO_OK;
     const int nc = c->s.nc;
     double r = 0.0, t = 0.0;
@@ -2038,14 +2053,14 @@ void set_const(StepConst &s, const meso_step_params &p) {
 }
 
 int sync_flags(meso_hip_ctx *c) {
-    HIP_TRY(c, hipMemcpyAsync(c->flags_host, c->flags, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->flags_host, c->flags, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MESO_OK;
 }
 
 int reset_flags(meso_hip_ctx *c) {
     hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(1), 0, c->stream, c->flags, c->flags + 1, c->flags + 2,
-                       c->flags + 3);
+                       c->flags + 3, c->flags + 4, c->steps_done);
     HIP_TRY(c, hipGetLastError());
     return MESO_OK;
 }
@@ -2072,6 +2087,7 @@ int default_vchunk(int nv, int npatch, int ncu) {
 }
 
 void set_vchunk(meso_hip_ctx *c, int vchunk) {
+    ++c->kver;
     c->p_vchunk = vchunk;
     c->p_chunks = (c->s.nv + vchunk - 1) / vchunk;
     c->s.ntile_c = c->s.ntile_f = c->p_chunks;
@@ -2648,7 +2664,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         (rc = dalloc(c, &c->part_c, size_t(nc) * std::max(c->s.ntile_c, c->p_chunk_cap) * 4)) || (rc = dalloc(c, &c->fm, 5 * size_t(nc))) ||
         (rc = dalloc(c, &c->res_part, 8 * size_t((nc + BLOCK - 1) / BLOCK))) || (rc = dalloc(c, &c->res_out, 8)) ||
         (rc = dalloc(c, &c->face_eq, nf)) || (rc = dalloc(c, &c->face_eqf, nf)) || (rc = dalloc(c, &c->bnd, std::max(1, c->s.nb))) ||
-        (rc = dalloc(c, &c->flags, 4)) || (rc = dalloc(c, &c->fref_d, 2 * size_t(nv))) ||
+        (rc = dalloc(c, &c->flags, 5)) || (rc = dalloc(c, &c->fref_d, 2 * size_t(nv))) ||
         (rc = dalloc(c, &c->fref_f, 2 * size_t(nv))) || (rc = dalloc(c, &c->fref_d2, nv)) ||
         (rc = dalloc(c, &c->fref_f2, nv)))
         return fail(rc);
@@ -2672,7 +2688,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     }
     c->xi_h = xi_s;
     c->w_h = w_s;
-    if (hipHostMalloc((void **)&c->flags_host, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&c->flags_host, 5 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(set_err(nullptr, MESO_E_DEVICE, "hipHostMalloc failed"));
     if ((rc = reset_flags(c))) return fail(rc);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_err(nullptr, MESO_E_DEVICE, "upload failed"));
@@ -2683,6 +2699,8 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
 void meso_hip_destroy(meso_hip_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto &g : c->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
     for (auto &r : c->recs) {
         (void)hipEventDestroy(r.a);
         (void)hipEventDestroy(r.b);
@@ -2697,6 +2715,7 @@ int meso_hip_set_params(meso_hip_ctx *c, const meso_step_params *p) {
     if (!c || !p) return set_err(c, MESO_E_INVALID, "null argument");
     if (p->model != c->s.model || p->dim != c->s.D) return set_err(c, MESO_E_INVALID, "model/dimension cannot change");
     set_const(c->s, *p);
+    ++c->kver;
     return MESO_OK;
 }
 
@@ -2849,12 +2868,56 @@ int meso_hip_get_state(meso_hip_ctx *c, double *f, int32_t layout) {
     return MESO_OK;
 }
 
-// Launches step `i` of a run that started at steps_done (no synchronisation).
-static int step_issue(meso_hip_ctx *c, int i) {
+// Launches one step (no synchronisation).
+static int step_one(meso_hip_ctx *c) {
     KArgs a = make_args(c);
-    a.step_index = c->steps_done + i;
     const int rc = step_dispatch(c, a);
-    if (rc == MESO_OK) c->cur = 1 - c->cur;
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->flags + 4);
+    HIP_TRY(c, hipGetLastError());
+    c->cur = 1 - c->cur;
+    return MESO_OK;
+}
+
+// The executable graph of two steps from the current buffer parity, captured
+// (or captured again, after an argument change) on the context's stream.
+static int pair_graph(meso_hip_ctx *c) {
+    const int p = c->cur;
+    if (c->gexec[p] && c->gver[p] == c->kver) return MESO_OK;
+    if (c->gexec[p]) (void)hipGraphExecDestroy(c->gexec[p]);
+    c->gexec[p] = nullptr;
+    HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = step_one(c);
+    if (rc == MESO_OK) rc = step_one(c);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->cur = p;  // capture launches nothing
+    if (rc == MESO_OK && e != hipSuccess) rc = set_err(c, MESO_E_DEVICE, std::string("step capture: ") + hipGetErrorString(e));
+    if (rc == MESO_OK) {
+        const hipError_t ei = hipGraphInstantiate(&c->gexec[p], g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            c->gexec[p] = nullptr;
+            rc = set_err(c, MESO_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    if (g) (void)hipGraphDestroy(g);
+    if (rc == MESO_OK) c->gver[p] = c->kver;
+    return rc;
+}
+
+// Launches up to `left` steps (two as one graph replay when the context
+// replays captured pairs); returns the steps launched in *n.
+static int step_issue(meso_hip_ctx *c, int left, int *n) {
+    *n = 0;
+    if (c->graphs && !c->profiling && left >= 2) {
+        int rc = pair_graph(c);
+        if (rc) return rc;
+        HIP_TRY(c, hipGraphLaunch(c->gexec[c->cur], c->stream));
+        *n = 2;
+        return MESO_OK;
+    }
+    const int rc = step_one(c);
+    if (rc == MESO_OK) *n = 1;
     return rc;
 }
 
@@ -2884,10 +2947,16 @@ static int step_finish(meso_hip_ctx *c, int32_t nsteps, meso_step_report *rep) {
 
 int meso_hip_step(meso_hip_ctx *c, int32_t nsteps, meso_step_report *rep) {
     if (!c || nsteps < 0) return set_err(c, MESO_E_INVALID, "bad argument");
-    int rc;
-    for (int i = 0; i < nsteps; ++i)
-        if ((rc = step_issue(c, i))) return rc;
+    int rc, n = 0;
+    for (int i = 0; i < nsteps; i += n)
+        if ((rc = step_issue(c, nsteps - i, &n))) return rc;
     return step_finish(c, nsteps, rep);
+}
+
+int meso_hip_set_graphs(meso_hip_ctx *c, int32_t enable) {
+    if (!c) return set_err(c, MESO_E_INVALID, "null context");
+    c->graphs = enable != 0;
+    return MESO_OK;
 }
 
 int meso_hip_step_batch(meso_hip_ctx *const *ctxs, int32_t n, int32_t nsteps, meso_step_report *reps) {
@@ -2897,12 +2966,16 @@ int meso_hip_step_batch(meso_hip_ctx *const *ctxs, int32_t n, int32_t nsteps, me
     // round-robin issue: step i of every context before step i + 1 of any, each
     // on its context's stream; then one wait per context
     std::vector<int> rc(n, MESO_OK), issued(n, 0);
-    for (int i = 0; i < nsteps; ++i)
+    for (bool more = true; more;) {
+        more = false;
         for (int k = 0; k < n; ++k)
-            if (rc[k] == MESO_OK) {
-                rc[k] = step_issue(ctxs[k], i);
-                if (rc[k] == MESO_OK) ++issued[k];
+            if (rc[k] == MESO_OK && issued[k] < nsteps) {
+                int got = 0;
+                rc[k] = step_issue(ctxs[k], nsteps - issued[k], &got);
+                issued[k] += got;
+                more = more || (rc[k] == MESO_OK && issued[k] < nsteps);
             }
+    }
     int first = MESO_OK;
     for (int k = 0; k < n; ++k) {
         meso_step_report r{};
@@ -2942,7 +3015,6 @@ int meso_hip_autotune(meso_hip_ctx *c, int32_t trial_steps, int32_t *vchunk_out)
     for (int vc : cand) {
         set_vchunk(c, vc);
         KArgs a = make_args(c);
-        a.step_index = c->steps_done;
         if ((rc = step_dispatch(c, a))) break;  // warm-up
         HIP_TRY(c, hipEventRecord(e0, c->stream));
         for (int i = 0; i < trial_steps && rc == MESO_OK; ++i) rc = step_dispatch(c, a);

@@ -109,7 +109,7 @@ struct KArgs {
     int *first_bad;
     int *bad_step;
     int *halt;
-    int step_index;
+    int *step_ctr;              // steps since the initial state (device counter)
     // LDS-staged patch kernels (2D): update patches (interior cells) and
     // moment patches (all faces); the velocity range is split into p_chunks
     // chunks of p_vchunk velocities

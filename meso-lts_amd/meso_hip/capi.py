@@ -101,6 +101,7 @@ def lib() -> C.CDLL:
             "meso_hip_step": ([vp, C.c_int32, C.POINTER(StepReport)], C.c_int),
             "meso_hip_step_batch": ([C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(StepReport)], C.c_int),
             "meso_hip_autotune": ([vp, C.c_int32, _i], C.c_int),
+            "meso_hip_set_graphs": ([vp, C.c_int32], C.c_int),
             "meso_hip_set_params": ([vp, C.POINTER(StepParams)], C.c_int),
             "meso_hip_get_macros": ([vp, _d, _d, _d, _d], C.c_int),
             "meso_hip_get_flux_moments": ([vp, _d, _d, _d], C.c_int),
@@ -306,6 +307,10 @@ class Device:
         out = C.c_int32()
         self._check(lib().meso_hip_autotune(self._h, trial_steps, C.byref(out)))
         return out.value
+
+    def set_graphs(self, enable: bool):
+        """Replay captured step pairs (meso_hip_set_graphs)."""
+        self._check(lib().meso_hip_set_graphs(self._h, 1 if enable else 0))
 
     def set_params(self, p: StepParams):
         self._check(lib().meso_hip_set_params(self._h, C.byref(p)))
