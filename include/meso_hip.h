@@ -168,7 +168,7 @@ int meso_case_reload_settings(meso_case *c);
 /* Per-cell initial density, velocity, temperature from the cell-group patches
  * (fixedValue or fromFile; cdugks_shakhov.cpp:108-131, cdugks.cpp:69-84). */
 int meso_case_initial_patch(const meso_case *c, double *rho, double *u3, double *T);
-/* Integer / string settings the driver loop needs. */
+/* Integer / string settings meso's time loop reads (max-step, write-interval, case-name, ...). */
 int meso_case_setting_int(const meso_case *c, const char *key, int32_t dflt, int32_t *out);
 int meso_case_setting_double(const meso_case *c, const char *key, double dflt, double *out);
 /* Writes at most cap bytes (NUL included). */
