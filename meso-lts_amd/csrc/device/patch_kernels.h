@@ -458,10 +458,17 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                         Xd *= venkata_phi(dw, fc[d], mn[d], mx[d], omega, sn >= VSMALL ? 1.0e-10 : VSMALL);
                     }
                     S *dst = &fb[(d * maxf + lf) * VTP + vl];
-                    if (mode == meso_patch::SIDE_BOUNDARY) {
-                        *dst = S(dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd) : fc[d] + Xd);
-                    } else if (up) {
-                        *dst = S(fc[d] + Xd);
+                    // FP32 storage: the face values also go to HBM for the
+                    // update kernel (a wave's lanes are 64 consecutive
+                    // velocities of one face); interior dead-band values are
+                    // not needed there (no flux)
+                    S *fo = sizeof(S) == 4 ? (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h) + size_t(F->face) * nvp + v
+                                           : nullptr;
+                    if (mode == meso_patch::SIDE_BOUNDARY || up) {
+                        const S fv = S(mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
+                                                                                 : fc[d] + Xd);
+                        *dst = fv;
+                        if constexpr (sizeof(S) == 4) *fo = fv;
                     } else if (dead) {
                         atomicAdd(dst, S(0.5 * (fc[d] + Xd)));
                     }
@@ -547,7 +554,11 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
     const int f0 = a.up_face_ptr[p], nfp = a.up_face_ptr[p + 1] - f0;
     const int r0 = a.up_row_ptr[p], nr = a.up_row_ptr[p + 1] - r0;
     const int nown = a.up_nown[p];
-    const int maxi = a.up_maxi, maxf = a.up_maxf, maxr = a.up_maxr;
+    // FP32 storage: the face values f-bar+ come from the moment kernel (HBM),
+    // so only the owned cells' items and rows are staged
+    constexpr bool FBI = sizeof(S) == 4;
+    const int maxi = FBI ? a.up_maxo : a.up_maxi, maxf = a.up_maxf, maxr = FBI ? a.up_maxo : a.up_maxr;
+    const int nis = FBI ? nown : ni, nrs = FBI ? nown : nr;
     // LDS: items | faces | rows | values | face fluxes | face and cell records
     auto *items = reinterpret_cast<Item *>(lds_raw);
     auto *faces = reinterpret_cast<Face *>(items + maxi);
@@ -569,9 +580,9 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
     const double hdt = a.s.half_dt;
     const S *fin_g = (const S *)a.fin_g, *fin_h = (const S *)a.fin_h;
     S *fout_g = (S *)a.fout_g, *fout_h = (S *)a.fout_h;
-    stage_copy(up_items + i0, ni, items);
+    stage_copy(up_items + i0, nis, items);
     stage_copy(up_faces + f0, nfp, faces);
-    for (int r = threadIdx.x; r < nr; r += BLOCKP) rows[r] = a.up_rows[r0 + r];
+    for (int r = threadIdx.x; r < nrs; r += BLOCKP) rows[r] = a.up_rows[r0 + r];
     for (int lf = threadIdx.x; lf < nfp; lf += BLOCKP) {
         const int f = up_faces[f0 + lf].face;
         feq[lf] = UpdateRecs<S>::face_eq(a, f);
@@ -596,12 +607,12 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
         const LaneVel L = lane_vel<D>(a, v);
         const double xv[3] = {L.x, L.y, L.z};
         // A: stage values
-        stage_values<ND, S, VT>(rows, nr, fin_g, fin_h, nvp, v, L.ok, slot, vl, val, maxr);
-        __syncthreads();
+        stage_values<ND, S, VT>(rows, nrs, fin_g, fin_h, nvp, v, L.ok, slot, vl, val, maxr);
+        if constexpr (!FBI) __syncthreads();
         // B1: gradients; each face's half-step value f-bar+ on its upwind side's
         // lanes, written to the face's flux slot (interior dead-band lanes write
         // nothing: their flux is 0)
-        for (int s = slot; s < ni; s += NSLOT) {
+        if constexpr (!FBI) for (int s = slot; s < ni; s += NSLOT) {
             const Item *it = items + s;
             const int nface = it->nface;
             double fc[2], g[2][D], mn[2], mx[2];
@@ -645,7 +656,7 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
                 }
             }
         }
-        __syncthreads();
+        if constexpr (!FBI) __syncthreads();
         // B2: per face and velocity (all lanes busy): equilibrium mix of f-bar+,
         // the boundary value, flux (xi.n) A f_face into the same slot
         if (L.ok)
@@ -663,7 +674,8 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
                 double fm[2] = {0.0, 0.0};
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                    const double fbar = double(flx[(d * maxf + lf) * VT + vl]);
+                    const double fbar = FBI ? double((d == 0 ? (const S *)a.fbar_g : (const S *)a.fbar_h)[size_t(f) * nvp + v])
+                                            : double(flx[(d * maxf + lf) * VT + vl]);
                     const double eq = d == 0 ? gs : hs;
                     if constexpr (sizeof(S) == 8) fm[d] = (1.0 - mix) * fbar + mix * eq;
                     else fm[d] = fbar + mix * (eq - fbar);

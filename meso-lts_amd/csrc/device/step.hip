@@ -1581,6 +1581,7 @@ struct meso_hip_ctx {
         *mp_face_ptr = nullptr, *up_row_ptr = nullptr, *up_rows = nullptr, *mp_row_ptr = nullptr,
         *mp_rows = nullptr;
     void *crec = nullptr, *cell_tab = nullptr;
+    void *fbar[2] = {nullptr, nullptr};  // FP32 patch path: face values g, h [nf][nvp]
     int *slot_of = nullptr;  // meso's velocity index -> storage slot (null: identity)
     int *cf_ptr = nullptr, *cf_ids = nullptr, *nb_ptr = nullptr, *nb_ids = nullptr, *face_bnd = nullptr,
         *bnd_face = nullptr, *bnd_mark = nullptr, *mirror = nullptr;
@@ -1733,6 +1734,8 @@ KArgs make_args(meso_hip_ctx *c) {
     a.fin_h = c->f[i][1];
     a.fout_g = c->f[o][0];
     a.fout_h = c->f[o][1];
+    a.fbar_g = c->fbar[0];
+    a.fbar_h = c->fbar[1];
     a.rho_in = c->rho[i];
     a.u_in = c->u[i];
     a.T_in = c->T[i];
@@ -2143,9 +2146,13 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     const int pmax = UPD_QJ * NWP * IPWU;
     for (int P = std::min(env_int("MESO_HIP_PATCH_CELLS", 32), pmax); ok; P = P * 3 / 4) {
         ok = meso_patch::build_update_patches<D>(mv, pown, P, ups, perr);
-        // items | faces | rows | values | face fluxes | face and cell records (k_patch_update)
-        c->lds_up = rec(IPWU, ups.max_items, ups.max_faces) + r16(4 * size_t(ups.max_rows)) +
-                    r16(size_t(ND) * ups.max_rows * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
+        // items | faces | rows | values | face fluxes | face and cell records (k_patch_update);
+        // FP32 storage reads the face values the moment kernel wrote and
+        // stages the owned cells only
+        const int ui = storage == MESO_STORAGE_FP32 ? ups.max_owned : ups.max_items;
+        const int ur = storage == MESO_STORAGE_FP32 ? ups.max_owned : ups.max_rows;
+        c->lds_up = rec(IPWU, ui, ups.max_faces) + r16(4 * size_t(ur)) +
+                    r16(size_t(ND) * ur * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
                     (storage == MESO_STORAGE_FP32 ? update_rec_bytes<float, D>(ups.max_faces, ups.max_owned)
                                                   : update_rec_bytes<double, D>(ups.max_faces, ups.max_owned));
         if (c->lds_up <= cap_up || P <= 4) break;
@@ -2685,6 +2692,12 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             if ((rc = dalloc(c, &ct, size_t(nc) * 2 * c->nt * es))) return fail(rc);
             c->cell_tab = ct;
         }
+        if (storage == MESO_STORAGE_FP32)
+            for (int k = 0; k < c->nd; ++k) {
+                char *fb = nullptr;
+                if ((rc = dalloc(c, &fb, size_t(nf) * c->s.nvp * es))) return fail(rc);
+                c->fbar[k] = fb;
+            }
     }
     c->xi_h = xi_s;
     c->w_h = w_s;
