@@ -458,17 +458,16 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                         Xd *= venkata_phi(dw, fc[d], mn[d], mx[d], omega, sn >= VSMALL ? 1.0e-10 : VSMALL);
                     }
                     S *dst = &fb[(d * maxf + lf) * VTP + vl];
-                    // FP32 storage: the face values also go to HBM for the
-                    // update kernel (a wave's lanes are 64 consecutive
-                    // velocities of one face); interior dead-band values are
-                    // not needed there (no flux)
-                    S *fo = sizeof(S) == 4 ? (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h) + size_t(F->face) * nvp + v
-                                           : nullptr;
+                    // the face values also go to HBM for the update kernel
+                    // (an item's lanes are consecutive velocities of one
+                    // face); interior dead-band values are not needed there
+                    // (no flux)
+                    S *fo = (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h) + size_t(F->face) * nvp + v;
                     if (mode == meso_patch::SIDE_BOUNDARY || up) {
                         const S fv = S(mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
                                                                                  : fc[d] + Xd);
                         *dst = fv;
-                        if constexpr (sizeof(S) == 4) *fo = fv;
+                        *fo = fv;
                     } else if (dead) {
                         atomicAdd(dst, S(0.5 * (fc[d] + Xd)));
                     }
@@ -554,9 +553,9 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
     const int f0 = a.up_face_ptr[p], nfp = a.up_face_ptr[p + 1] - f0;
     const int r0 = a.up_row_ptr[p], nr = a.up_row_ptr[p + 1] - r0;
     const int nown = a.up_nown[p];
-    // FP32 storage: the face values f-bar+ come from the moment kernel (HBM),
-    // so only the owned cells' items and rows are staged
-    constexpr bool FBI = sizeof(S) == 4;
+    // the face values f-bar+ come from the moment kernel (HBM), so only the
+    // owned cells' items and rows are staged
+    constexpr bool FBI = true;
     const int maxi = FBI ? a.up_maxo : a.up_maxi, maxf = a.up_maxf, maxr = FBI ? a.up_maxo : a.up_maxr;
     const int nis = FBI ? nown : ni, nrs = FBI ? nown : nr;
     // LDS: items | faces | rows | values | face fluxes | face and cell records

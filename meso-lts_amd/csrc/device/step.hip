@@ -1581,7 +1581,7 @@ struct meso_hip_ctx {
         *mp_face_ptr = nullptr, *up_row_ptr = nullptr, *up_rows = nullptr, *mp_row_ptr = nullptr,
         *mp_rows = nullptr;
     void *crec = nullptr, *cell_tab = nullptr;
-    void *fbar[2] = {nullptr, nullptr};  // FP32 patch path: face values g, h [nf][nvp]
+    void *fbar[2] = {nullptr, nullptr};  // patch path: face values g, h [nf][nvp] (storage type)
     int *slot_of = nullptr;  // meso's velocity index -> storage slot (null: identity)
     int *cf_ptr = nullptr, *cf_ids = nullptr, *nb_ptr = nullptr, *nb_ids = nullptr, *face_bnd = nullptr,
         *bnd_face = nullptr, *bnd_mark = nullptr, *mirror = nullptr;
@@ -2146,11 +2146,10 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     const int pmax = UPD_QJ * NWP * IPWU;
     for (int P = std::min(env_int("MESO_HIP_PATCH_CELLS", 32), pmax); ok; P = P * 3 / 4) {
         ok = meso_patch::build_update_patches<D>(mv, pown, P, ups, perr);
-        // items | faces | rows | values | face fluxes | face and cell records (k_patch_update);
-        // FP32 storage reads the face values the moment kernel wrote and
-        // stages the owned cells only
-        const int ui = storage == MESO_STORAGE_FP32 ? ups.max_owned : ups.max_items;
-        const int ur = storage == MESO_STORAGE_FP32 ? ups.max_owned : ups.max_rows;
+        // items | faces | rows | values | face fluxes | face and cell records
+        // (k_patch_update, which reads the face values the moment kernel wrote
+        // and stages the owned cells only)
+        const int ui = ups.max_owned, ur = ups.max_owned;
         c->lds_up = rec(IPWU, ui, ups.max_faces) + r16(4 * size_t(ur)) +
                     r16(size_t(ND) * ur * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
                     (storage == MESO_STORAGE_FP32 ? update_rec_bytes<float, D>(ups.max_faces, ups.max_owned)
@@ -2692,12 +2691,11 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             if ((rc = dalloc(c, &ct, size_t(nc) * 2 * c->nt * es))) return fail(rc);
             c->cell_tab = ct;
         }
-        if (storage == MESO_STORAGE_FP32)
-            for (int k = 0; k < c->nd; ++k) {
-                char *fb = nullptr;
-                if ((rc = dalloc(c, &fb, size_t(nf) * c->s.nvp * es))) return fail(rc);
-                c->fbar[k] = fb;
-            }
+        for (int k = 0; k < c->nd; ++k) {
+            char *fb = nullptr;
+            if ((rc = dalloc(c, &fb, size_t(nf) * c->s.nvp * es))) return fail(rc);
+            c->fbar[k] = fb;
+        }
     }
     c->xi_h = xi_s;
     c->w_h = w_s;

@@ -124,8 +124,8 @@ struct KArgs {
     int p_chunks, p_vchunk;     // velocity chunks of the patch kernels and their size (a multiple of 64)
     int n_int_cells;            // interior cells: cell_list[0, n_int_cells)
     void *crec;                 // [nc] CellRec of the storage type
-    // FP32 storage with patch kernels: f-bar+ of every face, written by the
-    // moment kernel and read by the update kernel (face-major [nf][nvp])
+    // patch kernels: f-bar+ of every face, written by the moment kernel and
+    // read by the update kernel (face-major [nf][nvp], storage type)
     void *fbar_g, *fbar_h;
     void *cell_tab;             // [nc][2][nt] axis tables of the cell equilibria
     // FP32 storage keeps f - f_ref; f_ref is the equilibrium `ref` at rest.
