@@ -26,10 +26,13 @@
 #pragma once
 
 constexpr int BLOCKP = 512;  // threads of a patch block
-// Owned cells per thread slot of an update block: a patch owns at most
-// UPD_QJ * (BLOCKP / VT) cells (each slot keeps its cells' heat-flux partials
-// in registers over all tiles).
-constexpr int UPD_QJ = 2;
+// Update patches own at most UPD_CELLS cells; a block has (BLOCKP / 64) *
+// (64 / VT) thread slots, each owning upd_qj(VT) of them (and keeping their
+// heat-flux partials in registers over all tiles).
+constexpr int UPD_CELLS = 32;
+__host__ __device__ constexpr int upd_qj(int vt) {
+    return (UPD_CELLS + (BLOCKP / 64) * (64 / vt) - 1) / ((BLOCKP / 64) * (64 / vt));
+}
 
 // Per-cell constants of the update (written by k_cell_prep for interior cells).
 template <class S, int D>
@@ -263,13 +266,27 @@ __device__ __forceinline__ double cell_xs(const CellS<D> &s, const double *xv) {
     return r;
 }
 
-// Bytes of LDS the face and cell records of k_patch_update take (the f_ref
-// flux factor only for FP32 storage, which stores deviations from f_ref).
+// LDS layout of a k_patch_update block (byte offsets), shared by the kernel
+// and the host that sizes the launch: item records of the owned cells | face
+// records | stage rows | values | face fluxes | face equilibria | mix weights |
+// cell constants | f_ref flux factors (FP32 storage, which stores deviations
+// from f_ref).
+__host__ __device__ constexpr size_t lds_r16(size_t x) { return (x + 15) & ~size_t(15); }
 template <class S, int D>
-__host__ __device__ constexpr size_t update_rec_bytes(int maxf, int maxo) {
-    return size_t(maxf) * (sizeof(typename UpdateRecs<S>::Eq) + 8) +
-           size_t(maxo) * (sizeof(CellRec<S, D>) + (sizeof(S) == 4 ? sizeof(CellS<D>) : 0));
-}
+struct UpdateLds {
+    size_t faces, rows, val, flx, feq, fmix, crl, csl, bytes;
+    __host__ __device__ UpdateLds(int maxo, int maxf, int vt, int nd) {
+        faces = size_t(maxo) * sizeof(meso_patch::ItemT<D>);
+        rows = faces + size_t(maxf) * sizeof(meso_patch::FaceT<D>);
+        val = lds_r16(rows + 4 * size_t(maxo));
+        flx = val + size_t(nd) * maxo * vt * sizeof(S);
+        feq = lds_r16(flx + size_t(nd) * maxf * vt * sizeof(S));
+        fmix = feq + size_t(maxf) * sizeof(typename UpdateRecs<S>::Eq);
+        crl = fmix + size_t(maxf) * 8;
+        csl = crl + size_t(maxo) * sizeof(CellRec<S, D>);
+        bytes = csl + (sizeof(S) == 4 ? size_t(maxo) * sizeof(CellS<D>) : 0);
+    }
+};
 
 // Equilibrium pair of face f at the lane velocity from its record e, in the
 // storage form (deviation for FP32, full value for FP64), as FP64.
@@ -556,22 +573,22 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
     // the face values f-bar+ come from the moment kernel (HBM), so only the
     // owned cells' items and rows are staged
     constexpr bool FBI = true;
-    const int maxi = FBI ? a.up_maxo : a.up_maxi, maxf = a.up_maxf, maxr = FBI ? a.up_maxo : a.up_maxr;
-    const int nis = FBI ? nown : ni, nrs = FBI ? nown : nr;
-    // LDS: items | faces | rows | values | face fluxes | face and cell records
-    auto *items = reinterpret_cast<Item *>(lds_raw);
-    auto *faces = reinterpret_cast<Face *>(items + maxi);
-    int *rows = reinterpret_cast<int *>(faces + maxf);
-    S *val = reinterpret_cast<S *>(lds_raw + ((reinterpret_cast<unsigned char *>(rows + maxr) - lds_raw + 15) & ~15));
-    S *flx = val + size_t(ND) * maxr * VT;
-    // the face equilibria and mix weights of the patch faces and the update
-    // constants of the owned cells, copied once per block
+    const int maxi = a.up_maxo, maxf = a.up_maxf, maxr = a.up_maxo;
+    const int nis = nown, nrs = nown;
+    // LDS (UpdateLds): items | faces | rows | values | face fluxes | face
+    // equilibria and mix weights of the patch faces and update constants of
+    // the owned cells (copied once per block)
     using EqS = typename UpdateRecs<S>::Eq;
-    EqS *feq = reinterpret_cast<EqS *>(
-        lds_raw + ((reinterpret_cast<unsigned char *>(flx + size_t(ND) * maxf * VT) - lds_raw + 15) & ~15));
-    double *fmix = reinterpret_cast<double *>(feq + maxf);
-    CellRec<S, D> *crl = reinterpret_cast<CellRec<S, D> *>(fmix + maxf);
-    CellS<D> *csl = reinterpret_cast<CellS<D> *>(crl + a.up_maxo);
+    const UpdateLds<S, D> lay(maxi, maxf, VT, ND);
+    auto *items = reinterpret_cast<Item *>(lds_raw);
+    auto *faces = reinterpret_cast<Face *>(lds_raw + lay.faces);
+    int *rows = reinterpret_cast<int *>(lds_raw + lay.rows);
+    S *val = reinterpret_cast<S *>(lds_raw + lay.val);
+    S *flx = reinterpret_cast<S *>(lds_raw + lay.flx);
+    EqS *feq = reinterpret_cast<EqS *>(lds_raw + lay.feq);
+    double *fmix = reinterpret_cast<double *>(lds_raw + lay.fmix);
+    CellRec<S, D> *crl = reinterpret_cast<CellRec<S, D> *>(lds_raw + lay.crl);
+    CellS<D> *csl = reinterpret_cast<CellS<D> *>(lds_raw + lay.csl);
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int sub = lane / VT, vl = lane % VT;
     const int slot = wave * IPW + sub;
@@ -595,6 +612,7 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
     const int ntiles = (a.s.nv + VT - 1) / VT;
     const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
     // Shakhov heat-flux partials of this thread's owned cells over the chunk
+    constexpr int UPD_QJ = upd_qj(VT);
     double qreg[UPD_QJ][D];
 #pragma unroll
     for (int j = 0; j < UPD_QJ; ++j)

@@ -1910,7 +1910,7 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
     const StepConst &s = c->s;
     constexpr int WPB = BLOCK / 64;
     constexpr int VTM = sizeof(T) == 4 ? 64 : 32;  // velocities per tile: moments
-    constexpr int VTU = sizeof(T) == 4 ? 32 : 16;  // velocities per tile: update
+    constexpr int VTU = VTM;                        // velocities per tile: update
     constexpr int MAXF = D == 2 ? 4 : 6;
     const int fwblk = (s.nf + WPB - 1) / WPB;
     if (!c->mp_attr[LIM] && c->lds_mp > 65536) {
@@ -2105,11 +2105,9 @@ template <int D>
 bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<double> &na, int storage,
                    const std::vector<char> &pown, meso_patch::PatchSet<D> &ups, meso_patch::PatchSet<D> &mps) {
     const int nc = m->n_cells, nf = m->n_faces, nv = c->s.nv;
-    // velocities per tile (must match step_patch): the moment kernel uses 64
-    // for FP32 storage (one item per wave, records read through the scalar
-    // cache), the update kernel 32 (two items per wave, records in LDS);
-    // FP64 storage halves both
-    const int VTM = storage == MESO_STORAGE_FP32 ? 64 : 32, VTU = VTM / 2;
+    // velocities per tile (must match step_patch): 64 for FP32 storage (one
+    // item per wave), 32 for FP64 (two items per wave), in both patch kernels
+    const int VTM = storage == MESO_STORAGE_FP32 ? 64 : 32, VTU = VTM;
     const int IPWM = 64 / VTM, IPWU = 64 / VTU;
     const size_t es = storage == MESO_STORAGE_FP32 ? 4 : 8;
     const int ND = c->nd;
@@ -2142,18 +2140,15 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     // cells x 16^3 velocities); FP32 (32-velocity tiles) gains nothing from it
     const size_t cap_up = D == 3 && storage == MESO_STORAGE_FP64 ? size_t(lds_blk) : cap;
     bool ok = true;
-    // owned cells per update patch: at most UPD_QJ per thread slot
-    const int pmax = UPD_QJ * NWP * IPWU;
+    // owned cells per update patch: at most upd_qj per thread slot
+    const int pmax = upd_qj(VTU) * NWP * IPWU;
     for (int P = std::min(env_int("MESO_HIP_PATCH_CELLS", 32), pmax); ok; P = P * 3 / 4) {
         ok = meso_patch::build_update_patches<D>(mv, pown, P, ups, perr);
-        // items | faces | rows | values | face fluxes | face and cell records
-        // (k_patch_update, which reads the face values the moment kernel wrote
-        // and stages the owned cells only)
-        const int ui = ups.max_owned, ur = ups.max_owned;
-        c->lds_up = rec(IPWU, ui, ups.max_faces) + r16(4 * size_t(ur)) +
-                    r16(size_t(ND) * ur * VTU * es + size_t(ND) * ups.max_faces * VTU * es) +
-                    (storage == MESO_STORAGE_FP32 ? update_rec_bytes<float, D>(ups.max_faces, ups.max_owned)
-                                                  : update_rec_bytes<double, D>(ups.max_faces, ups.max_owned));
+        // the layout k_patch_update uses (owned cells only: it reads the face
+        // values the moment kernel wrote)
+        c->lds_up = storage == MESO_STORAGE_FP32
+                        ? UpdateLds<float, D>(ups.max_owned, ups.max_faces, VTU, ND).bytes
+                        : UpdateLds<double, D>(ups.max_owned, ups.max_faces, VTU, ND).bytes;
         if (c->lds_up <= cap_up || P <= 4) break;
     }
     const int fcap = env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
