@@ -30,6 +30,14 @@ constexpr int BLOCKP = 512;  // threads of a patch block
 // (64 / VT) thread slots, each owning upd_qj(VT) of them (and keeping their
 // heat-flux partials in registers over all tiles).
 constexpr int UPD_CELLS = 32;
+// Velocities per tile of the update kernel: as wide as the moment tiles (64
+// FP32, 32 FP64) in 2D; in 3D the face-flux rows (6 faces per cell) would
+// shrink the patches to ~18 cells at 64, so FP32 takes 32 there (32^3 hexes
+// x 16^3 velocities: update 6.7 -> 4.0 ms).
+template <class S, int D>
+__host__ __device__ constexpr int upd_vt() {
+    return sizeof(S) == 4 ? (D == 2 ? 64 : 32) : 32;
+}
 __host__ __device__ constexpr int upd_qj(int vt) {
     return (UPD_CELLS + (BLOCKP / 64) * (64 / vt) - 1) / ((BLOCKP / 64) * (64 / vt));
 }

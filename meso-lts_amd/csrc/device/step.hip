@@ -1910,7 +1910,7 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
     const StepConst &s = c->s;
     constexpr int WPB = BLOCK / 64;
     constexpr int VTM = sizeof(T) == 4 ? 64 : 32;  // velocities per tile: moments
-    constexpr int VTU = VTM;                        // velocities per tile: update
+    constexpr int VTU = upd_vt<T, D>();             // velocities per tile: update
     constexpr int MAXF = D == 2 ? 4 : 6;
     const int fwblk = (s.nf + WPB - 1) / WPB;
     if (!c->mp_attr[LIM] && c->lds_mp > 65536) {
@@ -2105,9 +2105,11 @@ template <int D>
 bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<double> &na, int storage,
                    const std::vector<char> &pown, meso_patch::PatchSet<D> &ups, meso_patch::PatchSet<D> &mps) {
     const int nc = m->n_cells, nf = m->n_faces, nv = c->s.nv;
-    // velocities per tile (must match step_patch): 64 for FP32 storage (one
-    // item per wave), 32 for FP64 (two items per wave), in both patch kernels
-    const int VTM = storage == MESO_STORAGE_FP32 ? 64 : 32, VTU = VTM;
+    // velocities per tile (must match step_patch): moment kernel 64 for FP32
+    // storage (one item per wave), 32 for FP64 (two items per wave); update
+    // kernel upd_vt
+    const int VTM = storage == MESO_STORAGE_FP32 ? 64 : 32;
+    const int VTU = storage == MESO_STORAGE_FP32 ? upd_vt<float, D>() : upd_vt<double, D>();
     const int IPWM = 64 / VTM, IPWU = 64 / VTU;
     const size_t es = storage == MESO_STORAGE_FP32 ? 4 : 8;
     const int ND = c->nd;
@@ -2134,11 +2136,7 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     (void)hipDeviceGetAttribute(&lds_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device);
     (void)hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device);
     const size_t cap = std::min<size_t>(size_t(lds_blk), size_t(std::max(lds_cu, lds_blk)) / 2);
-    // 3D FP64 update blocks take the whole LDS of a CU (one block per CU): the
-    // 16-velocity tiles then serve 32-cell patches instead of 9-cell ones,
-    // with 2.8 instead of 3.8 items per owned cell (18.4 -> 13.8 ms for 32^3
-    // cells x 16^3 velocities); FP32 (32-velocity tiles) gains nothing from it
-    const size_t cap_up = D == 3 && storage == MESO_STORAGE_FP64 ? size_t(lds_blk) : cap;
+    const size_t cap_up = cap;
     bool ok = true;
     // owned cells per update patch: at most upd_qj per thread slot
     const int pmax = upd_qj(VTU) * NWP * IPWU;
