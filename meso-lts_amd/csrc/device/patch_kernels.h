@@ -363,8 +363,16 @@ __device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn,
 
 // ------------------------------------------------------------- moments
 // Raw moments of f-bar+ per face of the moment patches, summed per velocity
-// chunk into part_f[face][chunk][col] (the layout k_face_eq reads).
-template <int MODEL, class S, int VT, int D, bool LIM>
+// chunk into part_f[face][tile][col] (the layout k_face_eq reads).
+//   CELLS false: face patches; every face of the patch gets all of its terms
+//     (tile = chunk), the items are the cells on either side.
+//   CELLS true: cell patches; the patch forms the gradient of its owned
+//     cells only and adds, per face, the terms of the velocities for which an
+//     owned cell is upwind (half of each dead-band term), into the slot of
+//     that side (tile = side * p_chunks + chunk; the slot of a side the patch
+//     does not own is written by the patch that owns it, a face with both
+//     sides owned puts its sum in slot 0 and zeros in slot 1).
+template <int MODEL, class S, int VT, int D, bool LIM, bool CELLS>
 __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.bad_count > 0) {
@@ -387,8 +395,11 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     const int chunk = b / np, p = b - chunk * np;
     const int i0 = a.mp_item_ptr[p], ni = a.mp_item_ptr[p + 1] - i0;
     const int f0 = a.mp_face_ptr[p], nfp = a.mp_face_ptr[p + 1] - f0;
-    const int r0 = a.mp_row_ptr[p], nr = a.mp_row_ptr[p + 1] - r0;
-    const int maxi = a.mp_maxi, maxf = a.mp_maxf, maxr = a.mp_maxr;
+    const int nown = CELLS ? a.mp_nown[p] : ni;
+    // cell patches stage the items only (the owned cells' LSQ neighbours are
+    // the cells across their faces)
+    const int r0 = a.mp_row_ptr[p], nr = CELLS ? ni : a.mp_row_ptr[p + 1] - r0;
+    const int maxi = a.mp_maxi, maxf = a.mp_maxf, maxr = CELLS ? a.mp_maxi : a.mp_maxr;
     // LDS: [items | faces] | rows | tile velocities | values | f-bar+ rows
     unsigned char *q = lds_raw;
     Records<IPW, D> rec;
@@ -418,7 +429,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     const double hdt = a.s.half_dt;
     const S *fin_g = (const S *)a.fin_g, *fin_h = (const S *)a.fin_h;
     // phase-C layout: thread = (face, velocity subset); FG faces per subset
-    const int FG = maxf > 32 ? 64 : 32;
+    const int FG = maxf > 64 ? 128 : maxf > 32 ? 64 : 32;
     const int my_face = threadIdx.x % FG, vs = threadIdx.x / FG, nvs = BLOCKP / FG;
     const bool face_act = my_face < nfp;
     double acc[NCOL];
@@ -440,7 +451,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         }
         __syncthreads();
         // B: gradients and f-bar+ of the patch faces
-        for (int s = slot; s < ni; s += NSLOT) {
+        for (int s = slot; s < nown; s += NSLOT) {
             const Item *it = rec.item(s);
             const int nface = it->nface;
             double fc[2], g[2][D], mn[2], mx[2];
@@ -547,8 +558,16 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         const int lf = r / NCOL, k = r - lf * NCOL;
         double sum = 0.0;
         for (int qq = 0; qq < nvs; ++qq) sum += red[(size_t(qq) * FG + lf) * NCOL + k];
-        const int f = mp_faces[f0 + lf].face;
-        a.part_f[(size_t(f) * a.s.ntile_f + chunk) * NCOL + k] = sum;
+        const Face &F = mp_faces[f0 + lf];
+        const size_t base = size_t(F.face) * a.s.ntile_f;
+        if constexpr (CELLS) {
+            const bool oo = F.own < nown, on = F.nei < nown;
+            if (oo) a.part_f[(base + chunk) * NCOL + k] = sum;
+            if (on && !oo) a.part_f[(base + a.p_chunks + chunk) * NCOL + k] = sum;
+            else if (oo && (on || F.own == F.nei)) a.part_f[(base + a.p_chunks + chunk) * NCOL + k] = 0.0;
+        } else {
+            a.part_f[(base + chunk) * NCOL + k] = sum;
+        }
     }
 }
 

@@ -1574,9 +1574,11 @@ struct meso_hip_ctx {
     int gver[2] = {-1, -1};
     size_t lds_up = 0, lds_mp = 0;
     bool upd_attr[2] = {false, false};  // the dynamic-LDS limit of k_patch_update<.., LIM> has been raised
-    bool mp_attr[2] = {false, false};   // the same for k_patch_moments
+    bool mp_attr[2][2] = {{false, false}, {false, false}};  // the same for k_patch_moments [cells][LIM]
+    bool mp_cells = false;              // cell-based moment patches (k_patch_moments<..., true>)
     const void *up_items = nullptr, *mp_items = nullptr;  // meso_patch::ItemT<D>
     const void *up_faces = nullptr, *mp_faces = nullptr;  // meso_patch::FaceT<D>
+    int *mp_nown = nullptr;
     int *up_item_ptr = nullptr, *up_face_ptr = nullptr, *up_nown = nullptr, *mp_item_ptr = nullptr,
         *mp_face_ptr = nullptr, *up_row_ptr = nullptr, *up_rows = nullptr, *mp_row_ptr = nullptr,
         *mp_rows = nullptr;
@@ -1774,6 +1776,7 @@ KArgs make_args(meso_hip_ctx *c) {
     a.mp_maxi = c->mp_maxi;
     a.mp_maxf = c->mp_maxf;
     a.mp_items = c->mp_items;
+    a.mp_nown = c->mp_nown;
     a.mp_faces = c->mp_faces;
     a.mp_item_ptr = c->mp_item_ptr;
     a.mp_face_ptr = c->mp_face_ptr;
@@ -1913,12 +1916,23 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
     constexpr int VTU = upd_vt<T, D>();             // velocities per tile: update
     constexpr int MAXF = D == 2 ? 4 : 6;
     const int fwblk = (s.nf + WPB - 1) / WPB;
-    if (!c->mp_attr[LIM] && c->lds_mp > 65536) {
-        HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_mp)));
-        c->mp_attr[LIM] = true;
+    if (c->mp_cells) {
+        if (!c->mp_attr[1][LIM] && c->lds_mp > 65536) {
+            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, true>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_mp)));
+            c->mp_attr[1][LIM] = true;
+        }
+        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, true>), c->n_mp * c->p_chunks, c->lds_mp,
+                 a);
+    } else {
+        if (!c->mp_attr[0][LIM] && c->lds_mp > 65536) {
+            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, false>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_mp)));
+            c->mp_attr[0][LIM] = true;
+        }
+        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, false>), c->n_mp * c->p_chunks,
+                 c->lds_mp, a);
     }
-    LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM>), c->n_mp * c->p_chunks, c->lds_mp, a);
     LAUNCH_SH(c, "face_eq", (k_face_eq<MODEL, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
     if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, D, T, LIM>), s.nb, 0, a);
     if (c->n_int > 0) {
@@ -2093,7 +2107,9 @@ void set_vchunk(meso_hip_ctx *c, int vchunk) {
     ++c->kver;
     c->p_vchunk = vchunk;
     c->p_chunks = (c->s.nv + vchunk - 1) / vchunk;
-    c->s.ntile_c = c->s.ntile_f = c->p_chunks;
+    c->s.ntile_c = c->p_chunks;
+    // cell-based moment patches keep a partial sum per face side and chunk
+    c->s.ntile_f = (c->mp_cells ? 2 : 1) * c->p_chunks;
     c->s.vt_c = c->s.vt_f = c->p_vchunk;
 }
 
@@ -2149,16 +2165,39 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
                         : UpdateLds<double, D>(ups.max_owned, ups.max_faces, VTU, ND).bytes;
         if (c->lds_up <= cap_up || P <= 4) break;
     }
-    const int fcap = env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
-    for (int F = std::min(fcap, 64); ok; F = F * 3 / 4) {
-        ok = meso_patch::build_moment_patches<D>(mv, F, mps, perr);
-        // [items | faces] | rows | tile velocities | values | f-bar+ rows (k_patch_moments)
-        const size_t a1 = rec(IPWM, mps.max_items, mps.max_faces) + ((4 * size_t(mps.max_rows) + 31) & ~size_t(31)) +
-                          size_t(VTM) * 32 + size_t(ND) * mps.max_rows * VTM * es +
-                          size_t(ND) * mps.max_faces * (VTM + 1) * es;
-        const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer [threads][ncol]
-        c->lds_mp = std::max(a1, a2);
-        if (c->lds_mp <= cap || F <= 4) break;
+    // moment patches: cell-based (each owned cell's gradient once) or
+    // face-based. Measured: cells win for FP64 (two items per wave; 200^2 x
+    // 101^2 moments 7.9 -> 6.9 ms) and in 3D (cube FP32 10.0 -> 9.6 ms);
+    // faces win for FP32 in 2D (one item per wave: 5.3 against 6.3 ms, the
+    // moment phase then spends a third of its threads idle on the ~2.4 faces
+    // per owned cell). MESO_HIP_MOMENT_CELLS=0/1 overrides.
+    c->mp_cells = env_int("MESO_HIP_MOMENT_CELLS", storage == MESO_STORAGE_FP32 && D == 2 ? 0 : 1) != 0;
+    if (c->mp_cells) {
+        const std::vector<char> all(nc, 1);
+        for (int P = std::min(env_int("MESO_HIP_MOMENT_PATCH_CELLS", 32), 64); ok; P = P * 3 / 4) {
+            ok = meso_patch::build_update_patches<D>(mv, all, P, mps, perr);
+            // [items | faces] | rows (the items) | tile velocities | values | f-bar+ rows
+            const size_t a1 = rec(IPWM, mps.max_items, mps.max_faces) + ((4 * size_t(mps.max_items) + 31) & ~size_t(31)) +
+                              size_t(VTM) * 32 + size_t(ND) * mps.max_items * VTM * es +
+                              size_t(ND) * mps.max_faces * (VTM + 1) * es;
+            const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer [threads][ncol]
+            c->lds_mp = std::max(a1, a2);
+            // the moment phase takes at most 128 faces per block
+            if ((c->lds_mp <= cap && mps.max_faces <= 128) || P <= 4) break;
+        }
+        ok = ok && mps.max_faces <= 128;
+    } else {
+        const int fcap = env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
+        for (int F = std::min(fcap, 64); ok; F = F * 3 / 4) {
+            ok = meso_patch::build_moment_patches<D>(mv, F, mps, perr);
+            // [items | faces] | rows | tile velocities | values | f-bar+ rows (k_patch_moments)
+            const size_t a1 = rec(IPWM, mps.max_items, mps.max_faces) + ((4 * size_t(mps.max_rows) + 31) & ~size_t(31)) +
+                              size_t(VTM) * 32 + size_t(ND) * mps.max_rows * VTM * es +
+                              size_t(ND) * mps.max_faces * (VTM + 1) * es;
+            const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer [threads][ncol]
+            c->lds_mp = std::max(a1, a2);
+            if (c->lds_mp <= cap || F <= 4) break;
+        }
     }
     if (!ok || c->lds_up > cap_up || c->lds_mp > cap || ups.n_patch == 0 || mps.n_patch == 0 ||
         ups.max_owned > pmax)
@@ -2205,7 +2244,8 @@ int upload_patches(meso_hip_ctx *c, const meso_patch::PatchSet<D> &ups, const me
         (rc = upload(c, &c->mp_rows, mps.rows.data(), mps.rows.size())) ||
         (rc = upload_rec(c, &c->mp_items, mps.items)) || (rc = upload_rec(c, &c->mp_faces, mps.faces)) ||
         (rc = upload(c, &c->mp_item_ptr, mps.item_ptr.data(), mps.item_ptr.size())) ||
-        (rc = upload(c, &c->mp_face_ptr, mps.face_ptr.data(), mps.face_ptr.size())))
+        (rc = upload(c, &c->mp_face_ptr, mps.face_ptr.data(), mps.face_ptr.size())) ||
+        (rc = upload(c, &c->mp_nown, mps.n_owned.data(), mps.n_owned.size())))
         return rc;
     return MESO_OK;
 }
@@ -2658,7 +2698,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     const int ncol = c->nd == 2 ? (D == 2 ? 11 : 17) : (D == 2 ? 6 : 10);
     if ((rc = dalloc(c, &c->rho_res, nc)) || (rc = dalloc(c, &c->u_res, 3 * size_t(nc))) ||
         (rc = dalloc(c, &c->T_res, nc)) || (rc = dalloc(c, &c->q_res, 3 * size_t(nc))) ||
-        (rc = dalloc(c, &c->part_f, size_t(nf) * std::max(c->s.ntile_f, c->p_chunk_cap) * ncol)) ||
+        (rc = dalloc(c, &c->part_f, size_t(nf) * std::max(c->s.ntile_f, (c->mp_cells ? 2 : 1) * c->p_chunk_cap) * ncol)) ||
         (rc = dalloc(c, &c->face_flux, size_t(nf) * (D + 2))) ||
         (rc = dalloc(c, &c->part_c, size_t(nc) * std::max(c->s.ntile_c, c->p_chunk_cap) * 4)) || (rc = dalloc(c, &c->fm, 5 * size_t(nc))) ||
         (rc = dalloc(c, &c->res_part, 8 * size_t((nc + BLOCK - 1) / BLOCK))) || (rc = dalloc(c, &c->res_out, 8)) ||

@@ -121,6 +121,7 @@ struct KArgs {
     const void *mp_items;
     const void *mp_faces;
     const int *mp_item_ptr, *mp_face_ptr, *mp_row_ptr, *mp_rows;
+    const int *mp_nown;         // cell-based moment patches: owned cells per patch
     int p_chunks, p_vchunk;     // velocity chunks of the patch kernels and their size (a multiple of 64)
     int n_int_cells;            // interior cells: cell_list[0, n_int_cells)
     void *crec;                 // [nc] CellRec of the storage type
