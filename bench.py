@@ -165,7 +165,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="instead: the batched cavity sweep of N seeded instances (FP32 storage)")
     ap.add_argument("--autotune", action="store_true", help="with --batch: autotune each instance first")
-    ap.add_argument("--graphs", action="store_true", help="with --batch: replay captured step pairs")
+    ap.add_argument("--graphs", action="store_true", help="replay captured step pairs (meso_hip_set_graphs)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
@@ -194,6 +194,7 @@ def main():
     for storage, tag in kinds:
         dev = capi.Device(cs, storage)
         dev.init_equilibrium()
+        dev.set_graphs(args.graphs)
         secs = time_steps(dev, args.steps, args.warmup)
         kt = kernel_profile(dev, min(args.steps, 5))
         macros[tag] = dev.macros()
@@ -259,7 +260,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": args.workload, "cells": nc, "faces": cs.n_faces, "velocities": nv,
-                   "distributions": nd, "storage": "fp32 g,h; fp64 moments/macros", "mesh_scale": args.scale},
+                   "distributions": nd, "storage": "fp32 g,h; fp64 moments/macros", "mesh_scale": args.scale,
+                   "graphs": bool(args.graphs)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": b_alg * units, "avg_launch_ms": dom_ms},
