@@ -572,13 +572,15 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
 }
 
 // ------------------------------------------------------------- update
-// Cells of the update patches: cell flux, f-tilde-plus update and the Shakhov
-// heat-flux partials part_c[cell][chunk][0..D-1]. The equilibrium records of
-// the patch faces and the update constants of the owned cells are copied to
-// LDS once per block (read per lane from HBM in every tile they would cost
-// address arithmetic and L2 latency).
+// Cells of the update patches: face fluxes from the f-bar+ rows the moment
+// kernel wrote (limited there when LIM), cell flux, f-tilde-plus update and
+// the Shakhov heat-flux partials part_c[cell][chunk][0..D-1]. The equilibrium
+// records of the patch faces and the update constants of the owned cells are
+// copied to LDS once per block (read per lane from HBM in every tile they
+// would cost address arithmetic and L2 latency). LIM only selects the
+// instance that pairs with the limited moment kernel.
 template <int MODEL, class S, int VT, bool TAB, int D, bool LIM>
-__global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
+__global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.halt) return;
     constexpr bool SH = MODEL == CDUGKS_SHAKHOV;
@@ -593,15 +595,13 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
     const int np = a.n_up;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int chunk = b / np, p = b - chunk * np;
-    const int i0 = a.up_item_ptr[p], ni = a.up_item_ptr[p + 1] - i0;
+    const int i0 = a.up_item_ptr[p];
     const int f0 = a.up_face_ptr[p], nfp = a.up_face_ptr[p + 1] - f0;
-    const int r0 = a.up_row_ptr[p], nr = a.up_row_ptr[p + 1] - r0;
+    const int r0 = a.up_row_ptr[p];
     const int nown = a.up_nown[p];
     // the face values f-bar+ come from the moment kernel (HBM), so only the
-    // owned cells' items and rows are staged
-    constexpr bool FBI = true;
+    // owned cells' items and rows are staged (they come first in the patch)
     const int maxi = a.up_maxo, maxf = a.up_maxf, maxr = a.up_maxo;
-    const int nis = nown, nrs = nown;
     // LDS (UpdateLds): items | faces | rows | values | face fluxes | face
     // equilibria and mix weights of the patch faces and update constants of
     // the owned cells (copied once per block)
@@ -623,9 +623,9 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
     const double hdt = a.s.half_dt;
     const S *fin_g = (const S *)a.fin_g, *fin_h = (const S *)a.fin_h;
     S *fout_g = (S *)a.fout_g, *fout_h = (S *)a.fout_h;
-    stage_copy(up_items + i0, nis, items);
+    stage_copy(up_items + i0, nown, items);
     stage_copy(up_faces + f0, nfp, faces);
-    for (int r = threadIdx.x; r < nrs; r += BLOCKP) rows[r] = a.up_rows[r0 + r];
+    for (int r = threadIdx.x; r < nown; r += BLOCKP) rows[r] = a.up_rows[r0 + r];
     for (int lf = threadIdx.x; lf < nfp; lf += BLOCKP) {
         const int f = up_faces[f0 + lf].face;
         feq[lf] = UpdateRecs<S>::face_eq(a, f);
@@ -650,58 +650,10 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
         const int v = t * VT + vl;
         const LaneVel L = lane_vel<D>(a, v);
         const double xv[3] = {L.x, L.y, L.z};
-        // A: stage values
-        stage_values<ND, S, VT>(rows, nrs, fin_g, fin_h, nvp, v, L.ok, slot, vl, val, maxr);
-        if constexpr (!FBI) __syncthreads();
-        // B1: gradients; each face's half-step value f-bar+ on its upwind side's
-        // lanes, written to the face's flux slot (interior dead-band lanes write
-        // nothing: their flux is 0)
-        if constexpr (!FBI) for (int s = slot; s < ni; s += NSLOT) {
-            const Item *it = items + s;
-            const int nface = it->nface;
-            double fc[2], g[2][D], mn[2], mx[2];
-#pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                fc[d] = double(val[(d * maxr + s) * VT + vl]);
-#pragma unroll
-                for (int k = 0; k < D; ++k) g[d][k] = 0.0;
-                if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d]);
-                if constexpr (LIM) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d]);
-            }
-            const double omega = LIM ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
-#pragma unroll
-            for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
-                if (k >= nface) continue;
-                const int e = it->face[k];
-                const int lf = e & 0xffff, mode = e >> 16;
-                const Face *F = faces + lf;
-                double sn = 0.0;
-#pragma unroll
-                for (int i = 0; i < D; ++i) sn += F->n[i] * xv[i];
-                const bool up = mode == meso_patch::SIDE_BOUNDARY ||
-                                (mode == meso_patch::SIDE_OWNER ? sn >= VSMALL : sn < -VSMALL);
-                const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
-                double ev[D];
-#pragma unroll
-                for (int i = 0; i < D; ++i) ev[i] = F->d[side][i] - xv[i] * hdt;
-#pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    double Xd = 0.0;
-#pragma unroll
-                    for (int i = 0; i < D; ++i) Xd += ev[i] * g[d][i];
-                    if constexpr (LIM) {
-                        double dw = 0.0;
-#pragma unroll
-                        for (int i = 0; i < D; ++i) dw += F->d[side][i] * g[d][i];
-                        Xd *= venkata_phi(dw, fc[d], mn[d], mx[d], omega, sn >= VSMALL ? 1.0e-10 : VSMALL);
-                    }
-                    // a boundary face's dead-band value 0.5 (f + f + X + X) is f + X
-                    if (up) flx[(d * maxf + lf) * VT + vl] = S(fc[d] + Xd);
-                }
-            }
-        }
-        if constexpr (!FBI) __syncthreads();
-        // B2: per face and velocity (all lanes busy): equilibrium mix of f-bar+,
+        // A: stage the owned cells' values (read in C only: no barrier before B)
+        stage_values<ND, S, VT>(rows, nown, fin_g, fin_h, nvp, v, L.ok, slot, vl, val, maxr);
+        // B: per face and velocity (all lanes busy): f-bar+ (written by the
+        // moment kernel), its equilibrium mix,
         // the boundary value, flux (xi.n) A f_face into the same slot
         if (L.ok)
             for (int lf = threadIdx.x / VT; lf < nfp; lf += BLOCKP / VT) {
@@ -718,8 +670,7 @@ __global__ __launch_bounds__(BLOCKP, LIM ? 3 : 4) void k_patch_update(KArgs a) {
                 double fm[2] = {0.0, 0.0};
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                    const double fbar = FBI ? double((d == 0 ? (const S *)a.fbar_g : (const S *)a.fbar_h)[size_t(f) * nvp + v])
-                                            : double(flx[(d * maxf + lf) * VT + vl]);
+                    const double fbar = double((d == 0 ? (const S *)a.fbar_g : (const S *)a.fbar_h)[size_t(f) * nvp + v]);
                     const double eq = d == 0 ? gs : hs;
                     if constexpr (sizeof(S) == 8) fm[d] = (1.0 - mix) * fbar + mix * eq;
                     else fm[d] = fbar + mix * (eq - fbar);
