@@ -15,7 +15,11 @@ OpenMP on this host, one step of the same workload).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scale s]
                        [--storage both|fp32|fp64] [--copy-calibration MIB]
-                       [--batch N [--autotune] [--graphs]]
+                       [--autotune] [--graphs] [--batch N]
+
+With --autotune each context first runs meso_hip_autotune (trial steps of the
+initial state, not committed) to choose its patch and velocity-chunk tiles.
+The JSON line records the tiles in use (config.tiles).
 """
 import argparse
 import json
@@ -164,7 +168,9 @@ def main():
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="instead: the batched cavity sweep of N seeded instances (FP32 storage)")
-    ap.add_argument("--autotune", action="store_true", help="with --batch: autotune each instance first")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run meso_hip_autotune before timing (off by default: the default tiles are "
+                         "deterministic, so that profiles of the same command see the same kernels)")
     ap.add_argument("--graphs", action="store_true", help="replay captured step pairs (meso_hip_set_graphs)")
     args = ap.parse_args()
     if args.copy_calibration:
@@ -191,9 +197,14 @@ def main():
     kinds = [(capi.STORAGE_FP32, "fp32"), (capi.STORAGE_FP64, "fp64")]
     if args.storage != "both":
         kinds = [k for k in kinds if k[1] == args.storage]
+    tiles = {}
     for storage, tag in kinds:
         dev = capi.Device(cs, storage)
         dev.init_equilibrium()
+        if args.autotune:
+            dev.autotune(3)
+        tiles[tag] = dev.tiles()
+        log(f"[bench] {tag} tiles: {tiles[tag]}")
         dev.set_graphs(args.graphs)
         secs = time_steps(dev, args.steps, args.warmup)
         kt = kernel_profile(dev, min(args.steps, 5))
@@ -261,7 +272,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "cells": nc, "faces": cs.n_faces, "velocities": nv,
                    "distributions": nd, "storage": "fp32 g,h; fp64 moments/macros", "mesh_scale": args.scale,
-                   "graphs": bool(args.graphs)},
+                   "graphs": bool(args.graphs), "autotune": bool(args.autotune), "tiles": tiles.get("fp32")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_launch": b_alg * units, "avg_launch_ms": dom_ms},

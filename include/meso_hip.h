@@ -229,13 +229,29 @@ int meso_hip_set_params(meso_hip_ctx *ctx, const meso_step_params *params);
  * context may appear once per call. No counterpart in the reference, which runs
  * one case per process (src/meso/exec.cpp:141-165). */
 int meso_hip_step_batch(meso_hip_ctx *const *ctxs, int32_t n, int32_t nsteps, meso_step_report *reports);
-/* Autotuner of the cell x velocity tile of the patch kernels: times
+/* Autotuner of the cell x velocity tiles of the patch kernels. It times
  * trial_steps steps of the current state (computed but not committed: the
- * state does not advance) for velocity chunks of half, the same and twice the
- * chunk count, keeps the fastest and returns its velocities per block
- * (0 where the per-cell kernels run). The default chunk already gives 16
- * blocks per CU where the velocity set allows. No counterpart in the reference. */
+ * state does not advance) for each candidate and keeps the fastest, in three
+ * stages: the moment patches (cell-based of 16 or 32 cells, face-based of 32
+ * or 64 faces), the update patches (16, 24 or 32 cells), the velocity chunk
+ * (half or twice the chunk count). Patch sets are rebuilt on the host and
+ * uploaded again. Returns the velocities per block (0 where the per-cell
+ * kernels run); meso_hip_tiles reports the rest. No counterpart in the
+ * reference. */
 int meso_hip_autotune(meso_hip_ctx *ctx, int32_t trial_steps, int32_t *vchunk);
+
+/* The tile shapes in use. */
+typedef struct meso_tile_config {
+    int32_t patches;          /* 1: patch kernels (else the per-cell kernels) */
+    int32_t update_cells;     /* largest update patch, owned cells */
+    int32_t moment_by_cells;  /* 1: cell-based moment patches, 0: face-based */
+    int32_t moment_size;      /* largest moment patch: owned cells or faces */
+    int32_t vchunk;           /* velocities per block */
+    int32_t update_vt;        /* velocities per tile, update kernel */
+    int32_t moment_vt;        /* velocities per tile, moment kernel */
+    int32_t reserved;
+} meso_tile_config;
+int meso_hip_tiles(const meso_hip_ctx *ctx, meso_tile_config *out);
 /* enable != 0: meso_hip_step and meso_hip_step_batch run steps in pairs as
  * replays of a hipGraph captured on the context's stream (one per buffer
  * parity, captured again after any change of parameters, state or tiles;

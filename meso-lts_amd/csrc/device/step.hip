@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "meso_hip.h"
@@ -1521,6 +1522,31 @@ __global__ void k_step_end(int *step_ctr) { *step_ctr += 1; }
 }  // namespace
 
 // ====================================================================== host
+// Host copy of the mesh arrays the patch builder reads, kept so that the
+// autotuner can rebuild the patch sets with other sizes.
+struct PatchSrc {
+    int nc = 0, nf = 0;
+    std::vector<int> face_cells, cf_ptr, cf_ids, nb_ptr, nb_ids;
+    std::vector<double> cell_pos, face_pos, face_area, face_normal, nb_a;
+    std::vector<char> pown;  // cells the update patches own
+    meso_patch::MeshView view() const {
+        meso_patch::MeshView mv;
+        mv.nc = nc;
+        mv.nf = nf;
+        mv.face_cells = face_cells.data();
+        mv.cell_pos = cell_pos.data();
+        mv.face_pos = face_pos.data();
+        mv.face_area = face_area.data();
+        mv.face_normal = face_normal.data();
+        mv.cf_ptr = cf_ptr.data();
+        mv.cf_ids = cf_ids.data();
+        mv.nb_ptr = nb_ptr.data();
+        mv.nb_ids = nb_ids.data();
+        mv.nb_a = nb_a.data();
+        return mv;
+    }
+};
+
 struct meso_hip_ctx {
     StepConst s{};
     int storage = MESO_STORAGE_FP64;
@@ -1531,6 +1557,11 @@ struct meso_hip_ctx {
     std::string err;
     int64_t bytes = 0;
     std::vector<void *> allocs;
+    std::unordered_map<void *, size_t> alloc_bytes;
+    PatchSrc psrc;
+    // requested patch sizes (0 / -1: the defaults): update cells, moment
+    // cells or faces, moment patch kind
+    int preq_up = 0, preq_mom = 0, preq_cells = -1;
     std::vector<meso_mark_desc> marks_h;
     std::vector<int> face_bnd_h, bnd_face_h;
     int n_marks = 0;
@@ -1576,6 +1607,7 @@ struct meso_hip_ctx {
     bool upd_attr[2] = {false, false};  // the dynamic-LDS limit of k_patch_update<.., LIM> has been raised
     bool mp_attr[2][2] = {{false, false}, {false, false}};  // the same for k_patch_moments [cells][LIM]
     bool mp_cells = false;              // cell-based moment patches (k_patch_moments<..., true>)
+    int mp_size = 0;                    // largest moment patch: owned cells or faces
     const void *up_items = nullptr, *mp_items = nullptr;  // meso_patch::ItemT<D>
     const void *up_faces = nullptr, *mp_faces = nullptr;  // meso_patch::FaceT<D>
     int *mp_nown = nullptr;
@@ -1644,9 +1676,26 @@ int dalloc(meso_hip_ctx *c, P **p, size_t n) {
     e = hipMemsetAsync(q, 0, bytes, c->stream);
     if (e != hipSuccess) return set_err(c, MESO_E_DEVICE, std::string("hipMemset: ") + hipGetErrorString(e));
     c->allocs.push_back(q);
+    c->alloc_bytes[q] = bytes;
     c->bytes += int64_t(bytes);
     *p = reinterpret_cast<P *>(q);
     return MESO_OK;
+}
+
+// Releases one allocation of the context (the caller has synchronised).
+template <class P>
+void dfree(meso_hip_ctx *c, P *&p) {
+    if (!p) return;
+    void *q = const_cast<void *>(reinterpret_cast<const void *>(p));
+    auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
+    if (it != c->allocs.end()) c->allocs.erase(it);
+    auto b = c->alloc_bytes.find(q);
+    if (b != c->alloc_bytes.end()) {
+        c->bytes -= int64_t(b->second);
+        c->alloc_bytes.erase(b);
+    }
+    (void)hipFree(q);
+    p = nullptr;
 }
 
 template <class P>
@@ -2118,9 +2167,11 @@ void set_vchunk(meso_hip_ctx *c, int vchunk) {
 // (half the LDS of a CU, so that two blocks share a CU). Sets the patch fields
 // of c; returns false when the mesh does not fit the patch records.
 template <int D>
-bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<double> &na, int storage,
-                   const std::vector<char> &pown, meso_patch::PatchSet<D> &ups, meso_patch::PatchSet<D> &mps) {
-    const int nc = m->n_cells, nf = m->n_faces, nv = c->s.nv;
+bool setup_patches(meso_hip_ctx *c, int storage, bool keep_chunk, meso_patch::PatchSet<D> &ups,
+                   meso_patch::PatchSet<D> &mps) {
+    const meso_patch::MeshView mv = c->psrc.view();
+    const std::vector<char> &pown = c->psrc.pown;
+    const int nc = mv.nc, nv = c->s.nv;
     // velocities per tile (must match step_patch): moment kernel 64 for FP32
     // storage (one item per wave), 32 for FP64 (two items per wave); update
     // kernel upd_vt
@@ -2131,20 +2182,6 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     const int ND = c->nd;
     const int ncol = D == 2 ? (ND == 2 ? 11 : 6) : (ND == 2 ? 17 : 10);
     constexpr int NWP = BLOCKP / 64;
-    meso_patch::MeshView mv;
-    mv.nc = nc;
-    mv.nf = nf;
-    mv.face_cells = m->face_cells;
-    mv.cell_pos = m->cell_pos;
-    mv.face_pos = m->face_pos;
-    mv.face_area = m->face_area;
-    mv.face_normal = m->face_normal;
-    mv.cf_ptr = m->cell_face_ptr;
-    mv.cf_ids = m->cell_face_ids;
-    mv.nb_ptr = m->nbr_ptr;
-    mv.nb_ids = m->nbr_ids;
-    mv.nb_a = na.data();
-    auto r16 = [](size_t x) { return (x + 15) & ~size_t(15); };
     std::string perr;
     const size_t si = sizeof(meso_patch::ItemT<D>), sf = sizeof(meso_patch::FaceT<D>);
     auto rec = [&](int ipw, int mi, int mf) { return ipw > 1 ? size_t(mi) * si + size_t(mf) * sf : size_t(0); };
@@ -2159,7 +2196,8 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     // 16 owned cells for FP64 in 3D (cube 32^3 x 16^3 update: 5.6 -> 5.2 ms),
     // else 32 (FP32 3D: 16 cells gave 4.5 against 4.0 ms)
     const int pdef = D == 3 && storage == MESO_STORAGE_FP64 ? 16 : 32;
-    for (int P = std::min(env_int("MESO_HIP_PATCH_CELLS", pdef), pmax); ok; P = P * 3 / 4) {
+    const int preq = c->preq_up > 0 ? c->preq_up : env_int("MESO_HIP_PATCH_CELLS", pdef);
+    for (int P = std::min(preq, pmax); ok; P = P * 3 / 4) {
         ok = meso_patch::build_update_patches<D>(mv, pown, P, ups, perr);
         // the layout k_patch_update uses (owned cells only: it reads the face
         // values the moment kernel wrote)
@@ -2174,12 +2212,14 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     // faces win for FP32 in 2D (one item per wave: 5.3 against 6.3 ms, the
     // moment phase then spends a third of its threads idle on the ~2.4 faces
     // per owned cell). MESO_HIP_MOMENT_CELLS=0/1 overrides.
-    c->mp_cells = env_int("MESO_HIP_MOMENT_CELLS", storage == MESO_STORAGE_FP32 && D == 2 ? 0 : 1) != 0;
+    c->mp_cells = c->preq_cells >= 0 ? c->preq_cells != 0
+                                      : env_int("MESO_HIP_MOMENT_CELLS", storage == MESO_STORAGE_FP32 && D == 2 ? 0 : 1) != 0;
     if (c->mp_cells) {
         const std::vector<char> all(nc, 1);
         // 16 owned cells in 3D (cube 32^3 x 16^3 moments: FP32 4.6 -> 3.8 ms,
         // FP64 5.7 -> 4.3 ms against 32), 32 in 2D
-        for (int P = std::min(env_int("MESO_HIP_MOMENT_PATCH_CELLS", D == 3 ? 16 : 32), 64); ok; P = P * 3 / 4) {
+        const int mreq = c->preq_mom > 0 ? c->preq_mom : env_int("MESO_HIP_MOMENT_PATCH_CELLS", D == 3 ? 16 : 32);
+        for (int P = std::min(mreq, 64); ok; P = P * 3 / 4) {
             ok = meso_patch::build_update_patches<D>(mv, all, P, mps, perr);
             // [items | faces] | rows (the items) | tile velocities | values | f-bar+ rows
             const size_t a1 = rec(IPWM, mps.max_items, mps.max_faces) + ((4 * size_t(mps.max_items) + 31) & ~size_t(31)) +
@@ -2192,7 +2232,7 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
         }
         ok = ok && mps.max_faces <= 128;
     } else {
-        const int fcap = env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
+        const int fcap = c->preq_mom > 0 ? c->preq_mom : env_int("MESO_HIP_PATCH_FACES", IPWM > 1 ? 64 : 32);
         for (int F = std::min(fcap, 64); ok; F = F * 3 / 4) {
             ok = meso_patch::build_moment_patches<D>(mv, F, mps, perr);
             // [items | faces] | rows | tile velocities | values | f-bar+ rows (k_patch_moments)
@@ -2208,13 +2248,20 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
         ups.max_owned > pmax)
         return false;
     c->patches = true;
-    int ncu = 256;
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
-    const int env_chunk = env_int("MESO_HIP_PCHUNK", 0);
-    set_vchunk(c, env_chunk > 0 ? std::max(64, (env_chunk + 63) / 64 * 64)
-                                : default_vchunk(nv, std::min(ups.n_patch, mps.n_patch), ncu));
-    // room for twice as many chunks (meso_hip_autotune tries them)
-    c->p_chunk_cap = std::min((nv + 63) / 64, 2 * c->p_chunks);
+    // the kernels' dynamic-LDS attributes are set again for the new sizes
+    c->upd_attr[0] = c->upd_attr[1] = false;
+    c->mp_attr[0][0] = c->mp_attr[0][1] = c->mp_attr[1][0] = c->mp_attr[1][1] = false;
+    if (keep_chunk) {
+        set_vchunk(c, c->p_vchunk);  // ntile_f follows the moment patch kind
+    } else {
+        int ncu = 256;
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+        const int env_chunk = env_int("MESO_HIP_PCHUNK", 0);
+        set_vchunk(c, env_chunk > 0 ? std::max(64, (env_chunk + 63) / 64 * 64)
+                                    : default_vchunk(nv, std::min(ups.n_patch, mps.n_patch), ncu));
+        // room for twice as many chunks (meso_hip_autotune tries them)
+        c->p_chunk_cap = std::min((nv + 63) / 64, 2 * c->p_chunks);
+    }
     c->n_up = ups.n_patch;
     c->up_maxi = ups.max_items;
     c->up_maxf = ups.max_faces;
@@ -2224,6 +2271,7 @@ bool setup_patches(meso_hip_ctx *c, const meso_mesh_desc *m, const std::vector<d
     c->mp_maxi = mps.max_items;
     c->mp_maxf = mps.max_faces;
     c->mp_maxr = mps.max_rows;
+    c->mp_size = c->mp_cells ? mps.max_owned : mps.max_faces;
     return true;
 }
 
@@ -2239,6 +2287,23 @@ int upload_rec(meso_hip_ctx *c, const void **dst, const std::vector<P> &v) {
 template <int D>
 int upload_patches(meso_hip_ctx *c, const meso_patch::PatchSet<D> &ups, const meso_patch::PatchSet<D> &mps) {
     int rc;
+    // the previous patch sets (autotuner), once no launch uses them
+    if (c->up_items) HIP_TRY(c, hipStreamSynchronize(c->stream));
+    dfree(c, c->up_items);
+    dfree(c, c->up_faces);
+    dfree(c, c->up_item_ptr);
+    dfree(c, c->up_face_ptr);
+    dfree(c, c->up_nown);
+    dfree(c, c->up_row_ptr);
+    dfree(c, c->up_rows);
+    dfree(c, c->mp_row_ptr);
+    dfree(c, c->mp_rows);
+    dfree(c, c->mp_items);
+    dfree(c, c->mp_faces);
+    dfree(c, c->mp_item_ptr);
+    dfree(c, c->mp_face_ptr);
+    dfree(c, c->mp_nown);
+    ++c->kver;
     if ((rc = upload_rec(c, &c->up_items, ups.items)) || (rc = upload_rec(c, &c->up_faces, ups.faces)) ||
         (rc = upload(c, &c->up_item_ptr, ups.item_ptr.data(), ups.item_ptr.size())) ||
         (rc = upload(c, &c->up_face_ptr, ups.face_ptr.data(), ups.face_ptr.size())) ||
@@ -2612,8 +2677,22 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             const int kind = m->marks[m->face_mark[f]].type;
             if (kind == MESO_BC_SYMMETRY || kind == MESO_BC_SLIP_WALL) pown[m->face_cells[2 * f]] = 0;
         }
-        const bool ok = D == 2 ? setup_patches<2>(c, m, na, storage, pown, ups2, mps2)
-                               : setup_patches<3>(c, m, na, storage, pown, ups3, mps3);
+        PatchSrc &ps = c->psrc;
+        ps.nc = nc;
+        ps.nf = nf;
+        ps.face_cells.assign(m->face_cells, m->face_cells + 2 * size_t(nf));
+        ps.cf_ptr.assign(m->cell_face_ptr, m->cell_face_ptr + nc + 1);
+        ps.cf_ids.assign(m->cell_face_ids, m->cell_face_ids + m->n_cell_faces);
+        ps.nb_ptr.assign(m->nbr_ptr, m->nbr_ptr + nc + 1);
+        ps.nb_ids.assign(m->nbr_ids, m->nbr_ids + m->n_nbr);
+        ps.cell_pos.assign(m->cell_pos, m->cell_pos + 3 * size_t(nc));
+        ps.face_pos.assign(m->face_pos, m->face_pos + 3 * size_t(nf));
+        ps.face_area.assign(m->face_area, m->face_area + nf);
+        ps.face_normal.assign(m->face_normal, m->face_normal + 3 * size_t(nf));
+        ps.nb_a = na;
+        ps.pown = pown;
+        const bool ok = D == 2 ? setup_patches<2>(c, storage, false, ups2, mps2)
+                               : setup_patches<3>(c, storage, false, ups3, mps3);
         if (ok) {
             // cell lists: patch cells first (k_cell_prep), then the mirror-boundary cells (k_cell_update)
             t_list.clear();
@@ -2703,7 +2782,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     const int ncol = c->nd == 2 ? (D == 2 ? 11 : 17) : (D == 2 ? 6 : 10);
     if ((rc = dalloc(c, &c->rho_res, nc)) || (rc = dalloc(c, &c->u_res, 3 * size_t(nc))) ||
         (rc = dalloc(c, &c->T_res, nc)) || (rc = dalloc(c, &c->q_res, 3 * size_t(nc))) ||
-        (rc = dalloc(c, &c->part_f, size_t(nf) * std::max(c->s.ntile_f, (c->mp_cells ? 2 : 1) * c->p_chunk_cap) * ncol)) ||
+        (rc = dalloc(c, &c->part_f, size_t(nf) * std::max(c->s.ntile_f, 2 * c->p_chunk_cap) * ncol)) ||
         (rc = dalloc(c, &c->face_flux, size_t(nf) * (D + 2))) ||
         (rc = dalloc(c, &c->part_c, size_t(nc) * std::max(c->s.ntile_c, c->p_chunk_cap) * 4)) || (rc = dalloc(c, &c->fm, 5 * size_t(nc))) ||
         (rc = dalloc(c, &c->res_part, 8 * size_t((nc + BLOCK - 1) / BLOCK))) || (rc = dalloc(c, &c->res_out, 8)) ||
@@ -3039,51 +3118,114 @@ int meso_hip_step_batch(meso_hip_ctx *const *ctxs, int32_t n, int32_t nsteps, me
     return first;
 }
 
+// Rebuilds and uploads the patch sets for the requested sizes (velocity chunk
+// kept); MESO_E_INVALID when they do not fit.
+static int apply_patches(meso_hip_ctx *c, int up, int mom, int cells) {
+    c->preq_up = up;
+    c->preq_mom = mom;
+    c->preq_cells = cells;
+    if (c->s.D == 2) {
+        meso_patch::PatchSet<2> u, m;
+        if (!setup_patches<2>(c, c->storage, true, u, m)) return MESO_E_INVALID;
+        return upload_patches<2>(c, u, m);
+    }
+    meso_patch::PatchSet<3> u, m;
+    if (!setup_patches<3>(c, c->storage, true, u, m)) return MESO_E_INVALID;
+    return upload_patches<3>(c, u, m);
+}
+
+// Milliseconds of `steps` trial steps of the current state (computed, not
+// committed), after one warm-up step.
+static int time_trial(meso_hip_ctx *c, int steps, hipEvent_t e0, hipEvent_t e1, float *ms) {
+    KArgs a = make_args(c);
+    int rc = step_dispatch(c, a);
+    if (rc) return rc;
+    HIP_TRY(c, hipEventRecord(e0, c->stream));
+    for (int i = 0; i < steps && rc == MESO_OK; ++i) rc = step_dispatch(c, a);
+    if (rc) return rc;
+    HIP_TRY(c, hipEventRecord(e1, c->stream));
+    HIP_TRY(c, hipEventSynchronize(e1));
+    HIP_TRY(c, hipEventElapsedTime(ms, e0, e1));
+    return MESO_OK;
+}
+
 int meso_hip_autotune(meso_hip_ctx *c, int32_t trial_steps, int32_t *vchunk_out) {
     if (!c || trial_steps < 1) return set_err(c, MESO_E_INVALID, "bad argument");
-    if (!c->patches) {  // the per-cell kernels have no velocity chunk to choose
+    if (!c->patches) {  // the per-cell kernels have no tiles to choose
         if (vchunk_out) *vchunk_out = 0;
         return MESO_OK;
-    }
-    const int nv = c->s.nv, base = c->p_chunks;
-    std::vector<int> cand;
-    for (int k : {base / 2, base, 2 * base}) {
-        if (k < 1 || k > c->p_chunk_cap) continue;
-        const int vc = ((nv + k - 1) / k + 63) / 64 * 64;
-        if ((nv + vc - 1) / vc <= c->p_chunk_cap && std::find(cand.begin(), cand.end(), vc) == cand.end())
-            cand.push_back(vc);
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(c, hipEventCreate(&e0));
     HIP_TRY(c, hipEventCreate(&e1));
-    const int keep = c->p_vchunk;
-    int best = keep, rc = MESO_OK;
-    float best_ms = 0.0f;
+    int rc = MESO_OK;
     // trial steps read the current state and write the other buffer without
     // making it current, so the state is not advanced
-    for (int vc : cand) {
-        set_vchunk(c, vc);
-        KArgs a = make_args(c);
-        if ((rc = step_dispatch(c, a))) break;  // warm-up
-        HIP_TRY(c, hipEventRecord(e0, c->stream));
-        for (int i = 0; i < trial_steps && rc == MESO_OK; ++i) rc = step_dispatch(c, a);
-        if (rc) break;
-        HIP_TRY(c, hipEventRecord(e1, c->stream));
-        HIP_TRY(c, hipEventSynchronize(e1));
-        float ms = 0.0f;
-        HIP_TRY(c, hipEventElapsedTime(&ms, e0, e1));
-        if (best_ms == 0.0f || ms < best_ms) {
+    struct Req {
+        int up, mom, cells;
+    };
+    Req best{c->preq_up, c->preq_mom, c->preq_cells}, cur = best;
+    float best_ms = 0.0f, ms = 0.0f;
+    rc = time_trial(c, trial_steps, e0, e1, &best_ms);
+    auto consider = [&](Req r) {
+        if (rc) return;
+        const int ar = apply_patches(c, r.up, r.mom, r.cells);
+        cur = r;
+        if (ar == MESO_E_INVALID) return;  // does not fit: skipped
+        if ((rc = ar)) return;
+        if ((rc = time_trial(c, trial_steps, e0, e1, &ms))) return;
+        if (ms < best_ms) {
             best_ms = ms;
-            best = vc;
+            best = r;
         }
+    };
+    // 1: the moment patches (kind and size), 2: the update patches, 3: the
+    // velocity chunk; each stage keeps the fastest
+    for (Req r : {Req{best.up, 16, 1}, Req{best.up, 32, 1}, Req{best.up, 32, 0}, Req{best.up, 64, 0}}) consider(r);
+    const Req m = best;
+    for (int up : {16, 24, 32}) consider(Req{up, m.mom, m.cells});
+    if (rc == MESO_OK && (cur.up != best.up || cur.mom != best.mom || cur.cells != best.cells))
+        rc = apply_patches(c, best.up, best.mom, best.cells);
+    if (rc == MESO_OK) {
+        const int nv = c->s.nv, base = c->p_chunks, keep = c->p_vchunk;
+        int bvc = keep;
+        for (int k : {base / 2, 2 * base}) {
+            if (k < 1 || k > c->p_chunk_cap) continue;
+            const int vc = ((nv + k - 1) / k + 63) / 64 * 64;
+            if (vc == keep || (nv + vc - 1) / vc > c->p_chunk_cap) continue;
+            set_vchunk(c, vc);
+            if ((rc = time_trial(c, trial_steps, e0, e1, &ms))) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                bvc = vc;
+            }
+        }
+        set_vchunk(c, bvc);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    set_vchunk(c, rc == MESO_OK ? best : keep);
     if (rc) return rc;
     if ((rc = sync_flags(c))) return rc;
     if (c->flags_host[0] > 0) return set_err(c, MESO_E_NUMERIC, "NaN density or T<0 in a trial step");
     if (vchunk_out) *vchunk_out = c->p_vchunk;
+    return MESO_OK;
+}
+
+int meso_hip_tiles(const meso_hip_ctx *c, meso_tile_config *out) {
+    if (!c || !out) return MESO_E_INVALID;
+    meso_tile_config t{};
+    t.patches = c->patches ? 1 : 0;
+    if (c->patches) {
+        t.update_cells = c->up_maxo;
+        t.moment_by_cells = c->mp_cells ? 1 : 0;
+        t.moment_size = c->mp_size;
+        t.vchunk = c->p_vchunk;
+        const bool f32 = c->storage == MESO_STORAGE_FP32;
+        t.moment_vt = f32 ? 64 : 32;
+        t.update_vt = c->s.D == 2 ? (f32 ? upd_vt<float, 2>() : upd_vt<double, 2>())
+                                  : (f32 ? upd_vt<float, 3>() : upd_vt<double, 3>());
+    }
+    *out = t;
     return MESO_OK;
 }
 

@@ -54,6 +54,12 @@ class StepParams(C.Structure):
                 ("reserved", C.c_int32), ("venkata_k", C.c_double)]
 
 
+class TileConfig(C.Structure):
+    _fields_ = [("patches", C.c_int32), ("update_cells", C.c_int32), ("moment_by_cells", C.c_int32),
+                ("moment_size", C.c_int32), ("vchunk", C.c_int32), ("update_vt", C.c_int32),
+                ("moment_vt", C.c_int32), ("reserved", C.c_int32)]
+
+
 class StepReport(C.Structure):
     _fields_ = [("steps_done", C.c_int32), ("bad_cells", C.c_int32), ("first_bad_cell", C.c_int32),
                 ("reserved", C.c_int32)]
@@ -102,6 +108,7 @@ def lib() -> C.CDLL:
             "meso_hip_step_batch": ([C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(StepReport)], C.c_int),
             "meso_hip_autotune": ([vp, C.c_int32, _i], C.c_int),
             "meso_hip_set_graphs": ([vp, C.c_int32], C.c_int),
+            "meso_hip_tiles": ([vp, C.POINTER(TileConfig)], C.c_int),
             "meso_hip_set_params": ([vp, C.POINTER(StepParams)], C.c_int),
             "meso_hip_get_macros": ([vp, _d, _d, _d, _d], C.c_int),
             "meso_hip_get_flux_moments": ([vp, _d, _d, _d], C.c_int),
@@ -307,6 +314,12 @@ class Device:
         out = C.c_int32()
         self._check(lib().meso_hip_autotune(self._h, trial_steps, C.byref(out)))
         return out.value
+
+    def tiles(self):
+        """The tile shapes in use (meso_hip_tiles) as a dict."""
+        t = TileConfig()
+        self._check(lib().meso_hip_tiles(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in TileConfig._fields_ if k != "reserved"}
 
     def set_graphs(self, enable: bool):
         """Replay captured step pairs (meso_hip_set_graphs)."""

@@ -57,6 +57,7 @@ def test_null_arguments_are_rejected():
     assert L.meso_hip_device_bytes(None) == 0
     assert L.meso_hip_autotune(None, 1, None) == capi.MESO_E_INVALID
     assert L.meso_hip_set_graphs(None, 1) == capi.MESO_E_INVALID
+    assert L.meso_hip_tiles(None, None) == capi.MESO_E_INVALID
     assert L.meso_hip_step_batch(None, 1, 1, None) == capi.MESO_E_INVALID
     arr = (C.c_void_p * 2)(None, None)
     assert L.meso_hip_step_batch(arr, 2, 1, None) == capi.MESO_E_INVALID  # a null context

@@ -62,9 +62,14 @@ def test_autotune_keeps_state_and_parity(cases):
     orc.step(2)
     before = dev.get_state()
     m_before = dev.macros()
+    t0 = dev.tiles()
     vc = dev.autotune(2)
-    _say(f"    chosen velocities per block: {vc}")
+    t1 = dev.tiles()
+    _say(f"    tiles before {t0}\n    tiles after  {t1}")
     assert vc == 0 or (vc % 64 == 0 and vc > 0)
+    assert t1["vchunk"] == vc and t1["patches"] == t0["patches"]
+    if t1["patches"]:
+        assert 1 <= t1["update_cells"] <= 32 and t1["moment_size"] >= 1
     assert np.array_equal(dev.get_state(), before)
     for key in ("rho", "u", "T", "q"):
         assert np.array_equal(dev.macros()[key], m_before[key])
@@ -74,4 +79,29 @@ def test_autotune_keeps_state_and_parity(cases):
     for key in ("rho", "u", "T", "q"):
         err = np.abs(md[key] - mo[key]).max() / max(np.abs(mo[key]).max(), 1e-300)
         assert err < 1e-10, f"{key}: {err:.2e}"
+    dev.close()
+
+
+@pytest.mark.parametrize("name,storage", [("shak_kn1_quad10", capi.STORAGE_FP32), ("shak_hex3", capi.STORAGE_FP64),
+                                          ("shak_quad6_lim", capi.STORAGE_FP64)])
+def test_autotuned_tiles_keep_parity(case_factory, name, storage):
+    """After the autotuner rebuilt the patch sets (the tiles it chose), the
+    step still matches the oracle: FP64 to 1e-10, FP32 to 2e-6 rel. L2."""
+    cs = capi.Case(case_factory(name))
+    dev = capi.Device(cs, storage)
+    orc = Oracle(cs)
+    dev.init_equilibrium()
+    orc.init_equilibrium()
+    dev.autotune(1)
+    _say(f"[gpu] autotuned {name} storage={storage}: {dev.tiles()}")
+    dev.step(6)
+    orc.step(6)
+    md, mo = dev.macros(), orc.macros()
+    for key in ("rho", "u", "T", "q"):
+        if storage == capi.STORAGE_FP64:
+            err = np.abs(md[key] - mo[key]).max() / max(np.abs(mo[key]).max(), 1e-300)
+            assert err < 1e-10, f"{key}: {err:.2e}"
+        else:
+            err = np.linalg.norm(md[key] - mo[key]) / max(np.linalg.norm(mo[key]), 1e-300)
+            assert err < 2e-6, f"{key}: {err:.2e}"
     dev.close()
