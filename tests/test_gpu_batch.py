@@ -105,3 +105,15 @@ def test_autotuned_tiles_keep_parity(case_factory, name, storage):
             err = np.linalg.norm(md[key] - mo[key]) / max(np.linalg.norm(mo[key]), 1e-300)
             assert err < 2e-6, f"{key}: {err:.2e}"
     dev.close()
+
+
+def test_tiles_without_patches(case_factory):
+    """Velocity sets under 64 run the per-cell kernels: no tiles to report or
+    tune (meso_hip_tiles patches = 0, meso_hip_autotune returns 0)."""
+    cs = capi.Case(case_factory("bgk_quad8"))  # D2Q9
+    dev = capi.Device(cs, capi.STORAGE_FP32)
+    dev.init_equilibrium()
+    t = dev.tiles()
+    assert t["patches"] == 0 and t["vchunk"] == 0
+    assert dev.autotune(1) == 0
+    dev.close()
