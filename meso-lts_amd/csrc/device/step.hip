@@ -161,6 +161,15 @@ template <class T, bool LIM>
 __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restrict__ fd, int k, int f, int c,
                                                double sn, int v, const double xh[3], double eq_full, double mix) {
     const int nvp = a.s.nvp;
+    if constexpr (sizeof(T) == 8) {
+        // FP64 storage with the patch kernels: the moment kernel wrote this
+        // face's f-bar+ (the same reconstruction, unrounded); FP32 storage
+        // rounds it, so it reconstructs here in FP64
+        if (a.fbar_g) {
+            const T *fbr = k == 0 ? (const T *)a.fbar_g : (const T *)a.fbar_h;
+            return (1.0 - mix) * double(fbr[size_t(f) * nvp + v]) + mix * eq_full;
+        }
+    }
     const double fs = double(fd[size_t(c) * nvp + v]);
     double fb;
     const bool dead = !(sn >= VSMALL || sn < -VSMALL);
