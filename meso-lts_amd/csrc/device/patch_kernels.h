@@ -429,9 +429,11 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     const double hdt = a.s.half_dt;
     const S *fin_g = (const S *)a.fin_g, *fin_h = (const S *)a.fin_h;
     // phase-C layout: thread = (face, velocity subset); FG faces per subset
-    const int FG = maxf > 64 ? 128 : maxf > 32 ? 64 : 32;
+    // (FG = the patch's faces, nvs = BLOCKP / FG subsets; the last
+    // BLOCKP mod FG threads idle)
+    const int FG = max(nfp, 1);
     const int my_face = threadIdx.x % FG, vs = threadIdx.x / FG, nvs = BLOCKP / FG;
-    const bool face_act = my_face < nfp;
+    const bool face_act = vs < nvs;
     double acc[NCOL];
 #pragma unroll
     for (int k = 0; k < NCOL; ++k) acc[k] = 0.0;

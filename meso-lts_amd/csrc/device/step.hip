@@ -2215,19 +2215,18 @@ bool setup_patches(meso_hip_ctx *c, int storage, bool keep_chunk, meso_patch::Pa
                         : UpdateLds<double, D>(ups.max_owned, ups.max_faces, VTU, ND).bytes;
         if (c->lds_up <= cap_up || P <= 4) break;
     }
-    // moment patches: cell-based (each owned cell's gradient once) or
-    // face-based. Measured: cells win for FP64 (two items per wave; 200^2 x
-    // 101^2 moments 7.9 -> 6.9 ms) and in 3D (cube FP32 10.0 -> 9.6 ms);
-    // faces win for FP32 in 2D (one item per wave: 5.3 against 6.3 ms, the
-    // moment phase then spends a third of its threads idle on the ~2.4 faces
-    // per owned cell). MESO_HIP_MOMENT_CELLS=0/1 overrides.
-    c->mp_cells = c->preq_cells >= 0 ? c->preq_cells != 0
-                                      : env_int("MESO_HIP_MOMENT_CELLS", storage == MESO_STORAGE_FP32 && D == 2 ? 0 : 1) != 0;
+    // moment patches: cell-based (each owned cell's gradient once; the
+    // default) or face-based (MESO_HIP_MOMENT_CELLS=0). Measured on 200^2 x
+    // 101^2 moments: FP32 4.9 ms (16-cell patches) against 5.3 (32-face
+    // patches), FP64 6.9 (32 cells) against 7.9 (64 faces); cube 32^3 x 16^3
+    // FP32 3.5 ms against 4.6
+    c->mp_cells = c->preq_cells >= 0 ? c->preq_cells != 0 : env_int("MESO_HIP_MOMENT_CELLS", 1) != 0;
     if (c->mp_cells) {
         const std::vector<char> all(nc, 1);
-        // 16 owned cells in 3D (cube 32^3 x 16^3 moments: FP32 4.6 -> 3.8 ms,
-        // FP64 5.7 -> 4.3 ms against 32), 32 in 2D
-        const int mreq = c->preq_mom > 0 ? c->preq_mom : env_int("MESO_HIP_MOMENT_PATCH_CELLS", D == 3 ? 16 : 32);
+        // 16 owned cells (cube 32^3 x 16^3 moments: FP32 4.6 -> 3.8 ms, FP64
+        // 5.7 -> 4.3 ms against 32), 32 for FP64 in 2D (6.9 against 7.5 ms)
+        const int mdef = D == 2 && storage == MESO_STORAGE_FP64 ? 32 : 16;
+        const int mreq = c->preq_mom > 0 ? c->preq_mom : env_int("MESO_HIP_MOMENT_PATCH_CELLS", mdef);
         for (int P = std::min(mreq, 64); ok; P = P * 3 / 4) {
             ok = meso_patch::build_update_patches<D>(mv, all, P, mps, perr);
             // [items | faces] | rows (the items) | tile velocities | values | f-bar+ rows
