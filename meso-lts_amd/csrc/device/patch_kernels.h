@@ -657,8 +657,10 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
         // B: per face and velocity (all lanes busy): f-bar+ (written by the
         // moment kernel), its equilibrium mix,
         // the boundary value, flux (xi.n) A f_face into the same slot
-        if (L.ok)
-            for (int lf = threadIdx.x / VT; lf < nfp; lf += BLOCKP / VT) {
+        if (L.ok) {
+            constexpr int FS = BLOCKP / VT;  // faces per pass of the block
+            const S *fbg = (const S *)a.fbar_g, *fbh = (const S *)a.fbar_h;
+            auto face_flux = [&](const int lf, const S *fbv) {
                 const Face *F = faces + lf;
                 double sn = 0.0;
 #pragma unroll
@@ -672,7 +674,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                 double fm[2] = {0.0, 0.0};
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                    const double fbar = double((d == 0 ? (const S *)a.fbar_g : (const S *)a.fbar_h)[size_t(f) * nvp + v]);
+                    const double fbar = double(fbv[d]);
                     const double eq = d == 0 ? gs : hs;
                     if constexpr (sizeof(S) == 8) fm[d] = (1.0 - mix) * fbar + mix * eq;
                     else fm[d] = fbar + mix * (eq - fbar);
@@ -682,7 +684,30 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
 #pragma unroll
                 for (int d = 0; d < ND; ++d)
                     flx[(d * maxf + lf) * VT + vl] = S(dead && !bnd ? 0.0 : knA * fm[d]);
+            };
+            if constexpr (sizeof(S) == 4) {
+                // two faces per pass, both faces' f-bar+ loads issued before
+                // either is used (more loads in flight per wave)
+                for (int lf = threadIdx.x / VT; lf < nfp; lf += 2 * FS) {
+                    const bool two = lf + FS < nfp;
+                    const size_t o1 = size_t(faces[lf].face) * nvp + v;
+                    const size_t o2 = size_t(faces[two ? lf + FS : lf].face) * nvp + v;
+                    S b1[2], b2[2];
+                    b1[0] = fbg[o1];
+                    b2[0] = fbg[o2];
+                    b1[1] = SH ? fbh[o1] : S(0);
+                    b2[1] = SH ? fbh[o2] : S(0);
+                    face_flux(lf, b1);
+                    if (two) face_flux(lf + FS, b2);
+                }
+            } else {
+                for (int lf = threadIdx.x / VT; lf < nfp; lf += FS) {
+                    const size_t o = size_t(faces[lf].face) * nvp + v;
+                    const S b[2] = {fbg[o], SH ? fbh[o] : S(0)};
+                    face_flux(lf, b);
+                }
             }
+        }
         __syncthreads();
         // C: owned cells (this thread's are slot + j NSLOT, the same in every tile)
 #pragma nounroll
