@@ -685,27 +685,22 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                 for (int d = 0; d < ND; ++d)
                     flx[(d * maxf + lf) * VT + vl] = S(dead && !bnd ? 0.0 : knA * fm[d]);
             };
-            if constexpr (sizeof(S) == 4) {
-                // two faces per pass, both faces' f-bar+ loads issued before
-                // either is used (more loads in flight per wave)
-                for (int lf = threadIdx.x / VT; lf < nfp; lf += 2 * FS) {
-                    const bool two = lf + FS < nfp;
-                    const size_t o1 = size_t(faces[lf].face) * nvp + v;
-                    const size_t o2 = size_t(faces[two ? lf + FS : lf].face) * nvp + v;
-                    S b1[2], b2[2];
-                    b1[0] = fbg[o1];
-                    b2[0] = fbg[o2];
-                    b1[1] = SH ? fbh[o1] : S(0);
-                    b2[1] = SH ? fbh[o2] : S(0);
-                    face_flux(lf, b1);
-                    if (two) face_flux(lf + FS, b2);
+            // FP32: four faces per pass, all four faces' f-bar+ loads issued
+            // before any is used (more loads in flight per wave; 2 and 8 were
+            // slower); FP64 one (two were slower)
+            constexpr int NB = sizeof(S) == 4 ? 4 : 1;
+            for (int lf = threadIdx.x / VT; lf < nfp; lf += NB * FS) {
+                S fbv[NB][2];
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const int l = lf + k * FS < nfp ? lf + k * FS : lf;
+                    const size_t o = size_t(faces[l].face) * nvp + v;
+                    fbv[k][0] = fbg[o];
+                    fbv[k][1] = SH ? fbh[o] : S(0);
                 }
-            } else {
-                for (int lf = threadIdx.x / VT; lf < nfp; lf += FS) {
-                    const size_t o = size_t(faces[lf].face) * nvp + v;
-                    const S b[2] = {fbg[o], SH ? fbh[o] : S(0)};
-                    face_flux(lf, b);
-                }
+#pragma unroll
+                for (int k = 0; k < NB; ++k)
+                    if (lf + k * FS < nfp) face_flux(lf + k * FS, fbv[k]);
             }
         }
         __syncthreads();
