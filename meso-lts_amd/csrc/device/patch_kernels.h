@@ -704,8 +704,12 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             }
         }
         __syncthreads();
-        // C: owned cells (this thread's are slot + j NSLOT, the same in every tile)
-#pragma nounroll
+        // C: owned cells (this thread's are slot + j NSLOT, the same in every
+        // tile); FP32 2D two cells per step, so that the second cell's table
+        // and LDS reads overlap the first cell's arithmetic (FP64 and 3D:
+        // slower)
+        constexpr int CUNR = sizeof(S) == 4 && D == 2 ? 2 : 1;
+#pragma unroll CUNR
         for (int s = slot, j = 0; s < nown; s += NSLOT, ++j) {
             const Item *it = items + s;
             const int c = it->cell;
