@@ -467,22 +467,63 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
             }
             const double omega = LIM ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
             const double xv[3] = {X.x, X.y, X.z};
+            // the item's face records (entry, face id, normal, offset from
+            // this item) are read for all faces before any is used: one round
+            // trip through the scalar cache per item instead of three per
+            // face (not for the limited and FP64 3D instances, whose
+            // registers are full)
+            constexpr int MF = meso_patch::Lim<D>::MAXF;
+            constexpr bool PRE = !LIM && !(sizeof(S) == 8 && D == 3);
+            int fe[MF], fgl[MF];
+            double fnv[MF][D], fdv[MF][D];
+            if constexpr (PRE) {
 #pragma unroll
-            for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
+                for (int k = 0; k < MF; ++k) {
+                    fe[k] = it->face[k];
+                    // entries past nface are padding: read face 0 instead
+                    const Face *F = rec.face(k < nface ? fe[k] & 0xffff : 0);
+                    const int sd = (fe[k] >> 16) == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
+                    fgl[k] = F->face;
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        fnv[k][i] = F->n[i];
+                        fdv[k][i] = F->d[sd][i];
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < MF; ++k) {
                 if (k >= nface) continue;
-                const int e = it->face[k];
+                const int e = PRE ? fe[k] : it->face[k];
                 const int lf = e & 0xffff, mode = e >> 16;
-                const Face *F = rec.face(lf);
+                const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
+                double fn[D], fd[D];
+                int fg;
+                if constexpr (PRE) {
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        fn[i] = fnv[k][i];
+                        fd[i] = fdv[k][i];
+                    }
+                    fg = fgl[k];
+                } else {
+                    const Face *F = rec.face(lf);
+#pragma unroll
+                    for (int i = 0; i < D; ++i) {
+                        fn[i] = F->n[i];
+                        fd[i] = F->d[side][i];
+                    }
+                    fg = F->face;
+                }
                 double sn = 0.0;  // in 2D: + n_z xi_z, both 0
 #pragma unroll
-                for (int i = 0; i < D; ++i) sn += F->n[i] * xv[i];
+                for (int i = 0; i < D; ++i) sn += fn[i] * xv[i];
                 const bool up = mode == meso_patch::SIDE_BOUNDARY ||
                                 (mode == meso_patch::SIDE_OWNER ? sn >= VSMALL : sn < -VSMALL);
                 const bool dead = !(sn >= VSMALL) && !(sn < -VSMALL);
-                const int side = mode == meso_patch::SIDE_NEIGHBOUR ? 1 : 0;
                 double ev[D];
 #pragma unroll
-                for (int i = 0; i < D; ++i) ev[i] = F->d[side][i] - xv[i] * hdt;
+                for (int i = 0; i < D; ++i) ev[i] = fd[i] - xv[i] * hdt;
                 if (!ok) continue;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
@@ -492,7 +533,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                     if constexpr (LIM) {
                         double dw = 0.0;
 #pragma unroll
-                        for (int i = 0; i < D; ++i) dw += F->d[side][i] * g[d][i];
+                        for (int i = 0; i < D; ++i) dw += fd[i] * g[d][i];
                         Xd *= venkata_phi(dw, fc[d], mn[d], mx[d], omega, sn >= VSMALL ? 1.0e-10 : VSMALL);
                     }
                     S *dst = &fb[(d * maxf + lf) * VTP + vl];
@@ -500,7 +541,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                     // (an item's lanes are consecutive velocities of one
                     // face); interior dead-band values are not needed there
                     // (no flux)
-                    S *fo = (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h) + size_t(F->face) * nvp + v;
+                    S *fo = (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h) + size_t(fg) * nvp + v;
                     if (mode == meso_patch::SIDE_BOUNDARY || up) {
                         const S fv = S(mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
                                                                                  : fc[d] + Xd);
