@@ -298,13 +298,16 @@ struct UpdateLds {
 
 // Equilibrium pair of face f at the lane velocity from its record e, in the
 // storage form (deviation for FP32, full value for FP64), as FP64.
-template <int MODEL, class S, bool TAB, int D>
+// With PF the FP32 axis-table entries of the face come in tav (read earlier).
+template <int MODEL, class S, bool TAB, int D, bool PF = false>
 __device__ __forceinline__ void face_eq_rec(const KArgs &a, const typename UpdateRecs<S>::Eq &e, int f,
-                                            const LaneVel &L, double &gs, double &hs) {
+                                            const LaneVel &L, double &gs, double &hs, const float *tav = nullptr) {
     if constexpr (sizeof(S) == 4) {
         float g, h = 0.0f;
         if (MODEL == CDUGKS_SHAKHOV) {
-            if (TAB)
+            if constexpr (TAB && PF)
+                shakhov_tab_vals_f(e, tav[0], tav[1], tav[2], L.xf, D, a.s.K, L.gref, g, h);
+            else if (TAB)
                 shakhov_tab_f(e, reinterpret_cast<const float *>(a.face_tab) + size_t(f) * a.nt, L.vt, L.xf, D, a.s.K,
                               L.gref, g, h);
             else
@@ -701,7 +704,9 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
         if (L.ok) {
             constexpr int FS = BLOCKP / VT;  // faces per pass of the block
             const S *fbg = (const S *)a.fbar_g, *fbh = (const S *)a.fbar_h;
-            auto face_flux = [&](const int lf, const S *fbv) {
+            // FP32 Shakhov: the face axis-table entries are read with the f-bar+
+            constexpr bool PF = TAB && SH && sizeof(S) == 4;
+            auto face_flux = [&](const int lf, const S *fbv, const float *tav) {
                 const Face *F = faces + lf;
                 double sn = 0.0;
 #pragma unroll
@@ -710,7 +715,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                 const bool dead = !(sn >= VSMALL) && !(sn < -VSMALL);
                 const int f = F->face;
                 double gs, hs = 0.0;
-                face_eq_rec<MODEL, S, TAB, D>(a, feq[lf], f, L, gs, hs);
+                face_eq_rec<MODEL, S, TAB, D, PF>(a, feq[lf], f, L, gs, hs, tav);
                 const double mix = fmix[lf];
                 double fm[2] = {0.0, 0.0};
 #pragma unroll
@@ -726,22 +731,31 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                 for (int d = 0; d < ND; ++d)
                     flx[(d * maxf + lf) * VT + vl] = S(dead && !bnd ? 0.0 : knA * fm[d]);
             };
-            // FP32: four faces per pass, all four faces' f-bar+ loads issued
-            // before any is used (more loads in flight per wave; 2 and 8 were
-            // slower); FP64 one (two were slower)
+            // FP32: four faces per pass, all four faces' f-bar+ loads (and
+            // axis-table entries) issued before any is used (more loads in
+            // flight per wave; 2 and 8 were slower); FP64 one (two were slower)
             constexpr int NB = sizeof(S) == 4 ? 4 : 1;
+            const float *ftab = reinterpret_cast<const float *>(a.face_tab);
             for (int lf = threadIdx.x / VT; lf < nfp; lf += NB * FS) {
                 S fbv[NB][2];
+                float tav[NB][3];
 #pragma unroll
                 for (int k = 0; k < NB; ++k) {
                     const int l = lf + k * FS < nfp ? lf + k * FS : lf;
-                    const size_t o = size_t(faces[l].face) * nvp + v;
+                    const int fk = faces[l].face;
+                    const size_t o = size_t(fk) * nvp + v;
                     fbv[k][0] = fbg[o];
                     fbv[k][1] = SH ? fbh[o] : S(0);
+                    if constexpr (PF) {
+                        const float *tb = ftab + size_t(fk) * a.nt;
+                        tav[k][0] = tb[L.vt & 1023];
+                        tav[k][1] = tb[(L.vt >> 10) & 1023];
+                        tav[k][2] = D == 3 ? tb[(L.vt >> 20) & 1023] : 0.0f;
+                    }
                 }
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
-                    if (lf + k * FS < nfp) face_flux(lf + k * FS, fbv[k]);
+                    if (lf + k * FS < nfp) face_flux(lf + k * FS, fbv[k], tav[k]);
             }
         }
         __syncthreads();

@@ -84,14 +84,12 @@ __host__ __device__ inline EqF to_eqf(const EqParams &e) {
 //                 T_d[i] = expm1(B_ref X_i^2 - B (X_i - u_d)^2), a0 = expm1(lnAr)
 // so the per-velocity work is a few multiplies instead of an exponential.
 // vt packs the table offsets of a velocity's axis entries (10 bits each).
-__device__ __forceinline__ void shakhov_tab_f(const EqF &e, const float *tb, int vt, const float xi[3], int D, int K,
-                                              float gref, float &gs, float &hs) {
-    const float ax = tb[vt & 1023], ay = tb[(vt >> 10) & 1023];
+// FP32 Shakhov pair (deviation form) of one velocity from its axis-table
+// entries ax, ay (and az in 3D), read beforehand.
+__device__ __forceinline__ void shakhov_tab_vals_f(const EqF &e, float ax, float ay, float az, const float xi[3], int D,
+                                                   int K, float gref, float &gs, float &hs) {
     float t = fmaf(ax, ay, ax + ay);
-    if (D == 3) {
-        const float az = tb[(vt >> 20) & 1023];
-        t = fmaf(t, az, t + az);
-    }
+    if (D == 3) t = fmaf(t, az, t + az);
     const float em1 = fmaf(t, e.oa0, e.a0);
     const float c0 = xi[0] - e.u[0], c1 = xi[1] - e.u[1], c2 = xi[2] - e.u[2];
     const float cc = c0 * c0 + c1 * c1 + c2 * c2;
@@ -101,6 +99,13 @@ __device__ __forceinline__ void shakhov_tab_f(const EqF &e, const float *tb, int
     const float H = e.P2 * cq * ((ccr - float(D)) * float(K + 3 - D) - float(2 * K));
     gs = gref * fmaf(S, 1.0f + em1, em1);
     hs = gref * fmaf(em1, e.hk + H, e.dhk + H);
+}
+
+// The same, reading the entries from the table tb of one face or cell.
+__device__ __forceinline__ void shakhov_tab_f(const EqF &e, const float *tb, int vt, const float xi[3], int D, int K,
+                                              float gref, float &gs, float &hs) {
+    shakhov_tab_vals_f(e, tb[vt & 1023], tb[(vt >> 10) & 1023], D == 3 ? tb[(vt >> 20) & 1023] : 0.0f, xi, D, K, gref,
+                       gs, hs);
 }
 
 __device__ __forceinline__ void shakhov_tab_d(const EqParams &e, const double *tb, int vt, const double xi[3], int D,
