@@ -769,6 +769,20 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             const Item *it = items + s;
             const int c = it->cell;
             const int nface = it->nface;
+            // FP32 Shakhov: the cell's axis-table entries (tables of g_S^n and
+            // g_S^{n+1}) are read first, so that their latency overlaps the
+            // flux sums
+            constexpr bool PFC = TAB && SH && sizeof(S) == 4;
+            float ta[2][3];
+            if constexpr (PFC) {
+                const float *tb = reinterpret_cast<const float *>(a.cell_tab) + size_t(c) * 2 * a.nt;
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    ta[q][0] = tb[q * a.nt + (L.vt & 1023)];
+                    ta[q][1] = tb[q * a.nt + ((L.vt >> 10) & 1023)];
+                    ta[q][2] = D == 3 ? tb[q * a.nt + ((L.vt >> 20) & 1023)] : 0.0f;
+                }
+            }
             double F[2] = {0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
@@ -796,9 +810,8 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                     F[1] += L.href_d * xs;
                     float g1, h1, g2, h2;
                     if (TAB) {
-                        const float *tb = reinterpret_cast<const float *>(a.cell_tab) + size_t(c) * 2 * a.nt;
-                        shakhov_tab_f(R.en, tb, L.vt, L.xf, D, a.s.K, L.gref, g1, h1);
-                        shakhov_tab_f(R.e, tb + a.nt, L.vt, L.xf, D, a.s.K, L.gref, g2, h2);
+                        shakhov_tab_vals_f(R.en, ta[0][0], ta[0][1], ta[0][2], L.xf, D, a.s.K, L.gref, g1, h1);
+                        shakhov_tab_vals_f(R.e, ta[1][0], ta[1][1], ta[1][2], L.xf, D, a.s.K, L.gref, g2, h2);
                     } else {
                         shakhov_devf(R.en, L.xf, D, a.s.K, L.gref, g1, h1);
                         shakhov_devf(R.e, L.xf, D, a.s.K, L.gref, g2, h2);
