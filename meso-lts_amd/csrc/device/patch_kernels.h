@@ -298,10 +298,10 @@ struct UpdateLds {
 
 // Equilibrium pair of face f at the lane velocity from its record e, in the
 // storage form (deviation for FP32, full value for FP64), as FP64.
-// With PF the FP32 axis-table entries of the face come in tav (read earlier).
+// With PF the axis-table entries of the face come in tav (read earlier).
 template <int MODEL, class S, bool TAB, int D, bool PF = false>
 __device__ __forceinline__ void face_eq_rec(const KArgs &a, const typename UpdateRecs<S>::Eq &e, int f,
-                                            const LaneVel &L, double &gs, double &hs, const float *tav = nullptr) {
+                                            const LaneVel &L, double &gs, double &hs, const S *tav = nullptr) {
     if constexpr (sizeof(S) == 4) {
         float g, h = 0.0f;
         if (MODEL == CDUGKS_SHAKHOV) {
@@ -320,7 +320,9 @@ __device__ __forceinline__ void face_eq_rec(const KArgs &a, const typename Updat
     } else {
         const double x[3] = {L.x, L.y, L.z};
         if (MODEL == CDUGKS_SHAKHOV) {
-            if (TAB)
+            if constexpr (TAB && PF)
+                shakhov_tab_vals_d(e, tav[0], tav[1], tav[2], x, D, a.s.K, gs, hs);
+            else if (TAB)
                 shakhov_tab_d(e, reinterpret_cast<const double *>(a.face_tab) + size_t(f) * a.nt, L.vt, x, D, a.s.K,
                               gs, hs);
             else
@@ -704,9 +706,9 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
         if (L.ok) {
             constexpr int FS = BLOCKP / VT;  // faces per pass of the block
             const S *fbg = (const S *)a.fbar_g, *fbh = (const S *)a.fbar_h;
-            // FP32 Shakhov: the face axis-table entries are read with the f-bar+
-            constexpr bool PF = TAB && SH && sizeof(S) == 4;
-            auto face_flux = [&](const int lf, const S *fbv, const float *tav) {
+            // Shakhov: the face axis-table entries are read with the f-bar+
+            constexpr bool PF = TAB && SH;
+            auto face_flux = [&](const int lf, const S *fbv, const S *tav) {
                 const Face *F = faces + lf;
                 double sn = 0.0;
 #pragma unroll
@@ -735,10 +737,10 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             // axis-table entries) issued before any is used (more loads in
             // flight per wave; 2 and 8 were slower); FP64 one (two were slower)
             constexpr int NB = sizeof(S) == 4 ? 4 : 1;
-            const float *ftab = reinterpret_cast<const float *>(a.face_tab);
+            const S *ftab = reinterpret_cast<const S *>(a.face_tab);
             for (int lf = threadIdx.x / VT; lf < nfp; lf += NB * FS) {
                 S fbv[NB][2];
-                float tav[NB][3];
+                S tav[NB][3];
 #pragma unroll
                 for (int k = 0; k < NB; ++k) {
                     const int l = lf + k * FS < nfp ? lf + k * FS : lf;
@@ -747,10 +749,10 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                     fbv[k][0] = fbg[o];
                     fbv[k][1] = SH ? fbh[o] : S(0);
                     if constexpr (PF) {
-                        const float *tb = ftab + size_t(fk) * a.nt;
+                        const S *tb = ftab + size_t(fk) * a.nt;
                         tav[k][0] = tb[L.vt & 1023];
                         tav[k][1] = tb[(L.vt >> 10) & 1023];
-                        tav[k][2] = D == 3 ? tb[(L.vt >> 20) & 1023] : 0.0f;
+                        tav[k][2] = D == 3 ? tb[(L.vt >> 20) & 1023] : S(0);
                     }
                 }
 #pragma unroll
@@ -769,18 +771,17 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             const Item *it = items + s;
             const int c = it->cell;
             const int nface = it->nface;
-            // FP32 Shakhov: the cell's axis-table entries (tables of g_S^n and
-            // g_S^{n+1}) are read first, so that their latency overlaps the
-            // flux sums
-            constexpr bool PFC = TAB && SH && sizeof(S) == 4;
-            float ta[2][3];
-            if constexpr (PFC) {
-                const float *tb = reinterpret_cast<const float *>(a.cell_tab) + size_t(c) * 2 * a.nt;
+            // Shakhov: the cell's axis-table entries (tables of g_S^n and
+            // g_S^{n+1}, in the storage type) are read first, so that their
+            // latency overlaps the flux sums
+            S ta[2][3];
+            if constexpr (TAB && SH) {
+                const S *tb = reinterpret_cast<const S *>(a.cell_tab) + size_t(c) * 2 * a.nt;
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
                     ta[q][0] = tb[q * a.nt + (L.vt & 1023)];
                     ta[q][1] = tb[q * a.nt + ((L.vt >> 10) & 1023)];
-                    ta[q][2] = D == 3 ? tb[q * a.nt + ((L.vt >> 20) & 1023)] : 0.0f;
+                    ta[q][2] = D == 3 ? tb[q * a.nt + ((L.vt >> 20) & 1023)] : S(0);
                 }
             }
             double F[2] = {0.0, 0.0};
@@ -839,9 +840,8 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                     for (int i = 0; i < D; ++i) qv[i] = tq * cv[i];
                 } else {
                     if (TAB) {
-                        const double *tb = reinterpret_cast<const double *>(a.cell_tab) + size_t(c) * 2 * a.nt;
-                        shakhov_tab_d(R.en, tb, L.vt, xv, D, a.s.K, gsn, hsn);
-                        shakhov_tab_d(R.e, tb + a.nt, L.vt, xv, D, a.s.K, gs, hs);
+                        shakhov_tab_vals_d(R.en, ta[0][0], ta[0][1], ta[0][2], xv, D, a.s.K, gsn, hsn);
+                        shakhov_tab_vals_d(R.e, ta[1][0], ta[1][1], ta[1][2], xv, D, a.s.K, gs, hs);
                     } else {
                         shakhov_eq<double>(R.en, xv, D, a.s.K, gsn, hsn);
                         shakhov_eq<double>(R.e, xv, D, a.s.K, gs, hs);

@@ -108,16 +108,25 @@ __device__ __forceinline__ void shakhov_tab_f(const EqF &e, const float *tb, int
                        gs, hs);
 }
 
-__device__ __forceinline__ void shakhov_tab_d(const EqParams &e, const double *tb, int vt, const double xi[3], int D,
-                                              int K, double &gs, double &hs) {
-    double gm = e.A * tb[vt & 1023] * tb[(vt >> 10) & 1023];
-    if (D == 3) gm *= tb[(vt >> 20) & 1023];
+// FP64 Shakhov pair of one velocity from its axis-table entries ax, ay (and
+// az in 3D), read beforehand.
+__device__ __forceinline__ void shakhov_tab_vals_d(const EqParams &e, double ax, double ay, double az,
+                                                   const double xi[3], int D, int K, double &gs, double &hs) {
+    double gm = e.A * ax * ay;
+    if (D == 3) gm *= az;
     const double c0 = xi[0] - e.u[0], c1 = xi[1] - e.u[1], c2 = xi[2] - e.u[2];
     const double cc = c0 * c0 + c1 * c1 + c2 * c2;
     const double cq = c0 * e.q[0] + c1 * e.q[1] + c2 * e.q[2];
     const double ccr = cc * e.iRT;
     gs = gm * (1.0 + e.P1 * cq * (ccr - double(D) - 2.0));
     hs = gm * (e.hk + e.P2 * cq * ((ccr - double(D)) * double(K + 3 - D) - double(2 * K)));
+}
+
+// The same, reading the entries from the table tb of one face or cell.
+__device__ __forceinline__ void shakhov_tab_d(const EqParams &e, const double *tb, int vt, const double xi[3], int D,
+                                              int K, double &gs, double &hs) {
+    shakhov_tab_vals_d(e, tb[vt & 1023], tb[(vt >> 10) & 1023], D == 3 ? tb[(vt >> 20) & 1023] : 0.0, xi, D, K, gs,
+                       hs);
 }
 
 __device__ __forceinline__ void shakhov_devf(const EqF &e, const float xi[3], int D, int K, float gref, float &gs,
