@@ -706,7 +706,8 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
         if (L.ok) {
             constexpr int FS = BLOCKP / VT;  // faces per pass of the block
             const S *fbg = (const S *)a.fbar_g, *fbh = (const S *)a.fbar_h;
-            // Shakhov: the face axis-table entries are read with the f-bar+
+            // Shakhov with axis tables: each face's table entries are read with
+            // its f-bar+ values (PF)
             constexpr bool PF = TAB && SH;
             auto face_flux = [&](const int lf, const S *fbv, const S *tav) {
                 const Face *F = faces + lf;
