@@ -451,7 +451,10 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         const double4 X = a.vel_d[ok ? v : 0];
         // A: stage values and tile velocities, clear the face rows
         stage_values<ND, S, VT>(rows, nr, fin_g, fin_h, nvp, v, ok, slot, vl, val, maxr);
-        if (threadIdx.x < VT) vtile[threadIdx.x] = ok ? X : make_double4(0.0, 0.0, 0.0, 0.0);
+        // in 2D the z slot of a tile velocity carries |xi|^2 (heat-flux moments)
+        if (threadIdx.x < VT)
+            vtile[threadIdx.x] = ok ? (D == 2 ? make_double4(X.x, X.y, X.x * X.x + X.y * X.y, X.w) : X)
+                                    : make_double4(0.0, 0.0, 0.0, 0.0);
         for (int r = threadIdx.x; r < nfp * VTP; r += BLOCKP) {
             fb[r] = S(0);
             if (SH) fb[maxf * VTP + r] = S(0);
@@ -564,7 +567,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
             for (int vv = vs; vv < VT; vv += nvs) {
                 const double4 Y = vtile[vv];
                 const double wg = Y.w * double(fb[my_face * VTP + vv]);
-                const double wgx = wg * Y.x, wgy = wg * Y.y, wgz = wg * Y.z;
+                const double wgx = wg * Y.x, wgy = wg * Y.y, wgz = D == 3 ? wg * Y.z : 0.0;
                 acc[Col<D>::M0] += wg;
                 acc[Col<D>::M1] += wgx;
                 acc[Col<D>::M1 + 1] += wgy;
@@ -583,7 +586,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                 }
                 if (SH) {
                     const double wh = Y.w * double(fb[(maxf + my_face) * VTP + vv]);
-                    const double wgk = wg * (Y.x * Y.x + Y.y * Y.y + Y.z * Y.z);
+                    const double wgk = wg * (D == 2 ? Y.z : Y.x * Y.x + Y.y * Y.y + Y.z * Y.z);
                     acc[Col<D>::M3] += wgk * Y.x;
                     acc[Col<D>::M3 + 1] += wgk * Y.y;
                     if (D == 3) acc[Col<D>::M3 + 2] += wgk * Y.z;
