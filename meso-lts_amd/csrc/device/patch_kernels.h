@@ -164,6 +164,11 @@ struct Records {
     __device__ __forceinline__ const meso_patch::FaceT<D> *face(int lf) const { return faces + lf; }
 };
 
+// Bytes of the moment kernel's tile-velocity table: per velocity (x, y, z, w)
+// in 3D; in 2D w and its products with the moment monomials
+// (w, w x, w y, w x x, w x y, w y y, w |xi|^2 x, w |xi|^2 y).
+__host__ __device__ constexpr size_t mp_vtile_bytes(int D, int VT) { return size_t(VT) * (D == 2 ? 64 : 32); }
+
 // Bytes of LDS the records take (0 when they are read from HBM).
 template <int IPW, int D>
 __host__ __device__ constexpr size_t record_bytes(int maxi, int maxf) {
@@ -422,8 +427,9 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     }
     int *rows = reinterpret_cast<int *>(q);
     q += (size_t(maxr) * 4 + 31) & ~size_t(31);
-    double4 *vtile = reinterpret_cast<double4 *>(q);
-    S *val = reinterpret_cast<S *>(vtile + VT);
+    constexpr int VW = D == 2 ? 8 : 4;  // doubles per tile velocity (mp_vtile_bytes)
+    double *vtd = reinterpret_cast<double *>(q);
+    S *val = reinterpret_cast<S *>(vtd + VT * VW);
     S *fb = val + size_t(ND) * maxr * VT;
     for (int r = threadIdx.x; r < nr; r += BLOCKP) rows[r] = a.mp_rows[r0 + r];
     const int lane = threadIdx.x % 64;
@@ -451,10 +457,23 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         const double4 X = a.vel_d[ok ? v : 0];
         // A: stage values and tile velocities, clear the face rows
         stage_values<ND, S, VT>(rows, nr, fin_g, fin_h, nvp, v, ok, slot, vl, val, maxr);
-        // in 2D the z slot of a tile velocity carries |xi|^2 (heat-flux moments)
-        if (threadIdx.x < VT)
-            vtile[threadIdx.x] = ok ? (D == 2 ? make_double4(X.x, X.y, X.x * X.x + X.y * X.y, X.w) : X)
-                                    : make_double4(0.0, 0.0, 0.0, 0.0);
+        if (threadIdx.x < VT) {
+            double *y = vtd + threadIdx.x * VW;
+            if constexpr (D == 2) {
+                // the weights of the moment sums: one FMA per moment and value
+                const double w = ok ? X.w : 0.0, x = X.x, z = X.y, kk = x * x + z * z;
+                y[0] = w;
+                y[1] = w * x;
+                y[2] = w * z;
+                y[3] = w * x * x;
+                y[4] = w * x * z;
+                y[5] = w * z * z;
+                y[6] = w * kk * x;
+                y[7] = w * kk * z;
+            } else {
+                *reinterpret_cast<double4 *>(y) = ok ? X : make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+        }
         for (int r = threadIdx.x; r < nfp * VTP; r += BLOCKP) {
             fb[r] = S(0);
             if (SH) fb[maxf * VTP + r] = S(0);
@@ -565,7 +584,26 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         // C: moments, thread = (face, velocity subset)
         if (face_act) {
             for (int vv = vs; vv < VT; vv += nvs) {
-                const double4 Y = vtile[vv];
+                const double *y = vtd + vv * VW;
+                if constexpr (D == 2) {
+                    const double g = double(fb[my_face * VTP + vv]);
+                    acc[Col<D>::M0] += y[0] * g;
+                    acc[Col<D>::M1] += y[1] * g;
+                    acc[Col<D>::M1 + 1] += y[2] * g;
+                    acc[Col<D>::M2] += y[3] * g;
+                    acc[Col<D>::M2 + 1] += y[4] * g;
+                    acc[Col<D>::M2 + 2] += y[5] * g;
+                    if (SH) {
+                        const double h = double(fb[(maxf + my_face) * VTP + vv]);
+                        acc[Col<D>::M3] += y[6] * g;
+                        acc[Col<D>::M3 + 1] += y[7] * g;
+                        acc[Col<D>::H0] += y[0] * h;
+                        acc[Col<D>::H1] += y[1] * h;
+                        acc[Col<D>::H1 + 1] += y[2] * h;
+                    }
+                    continue;
+                }
+                const double4 Y = *reinterpret_cast<const double4 *>(y);
                 const double wg = Y.w * double(fb[my_face * VTP + vv]);
                 const double wgx = wg * Y.x, wgy = wg * Y.y, wgz = D == 3 ? wg * Y.z : 0.0;
                 acc[Col<D>::M0] += wg;
@@ -586,7 +624,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                 }
                 if (SH) {
                     const double wh = Y.w * double(fb[(maxf + my_face) * VTP + vv]);
-                    const double wgk = wg * (D == 2 ? Y.z : Y.x * Y.x + Y.y * Y.y + Y.z * Y.z);
+                    const double wgk = wg * (Y.x * Y.x + Y.y * Y.y + Y.z * Y.z);
                     acc[Col<D>::M3] += wgk * Y.x;
                     acc[Col<D>::M3 + 1] += wgk * Y.y;
                     if (D == 3) acc[Col<D>::M3 + 2] += wgk * Y.z;

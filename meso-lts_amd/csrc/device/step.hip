@@ -2231,7 +2231,7 @@ bool setup_patches(meso_hip_ctx *c, int storage, bool keep_chunk, meso_patch::Pa
             ok = meso_patch::build_update_patches<D>(mv, all, P, mps, perr);
             // [items | faces] | rows (the items) | tile velocities | values | f-bar+ rows
             const size_t a1 = rec(IPWM, mps.max_items, mps.max_faces) + ((4 * size_t(mps.max_items) + 31) & ~size_t(31)) +
-                              size_t(VTM) * 32 + size_t(ND) * mps.max_items * VTM * es +
+                              mp_vtile_bytes(D, VTM) + size_t(ND) * mps.max_items * VTM * es +
                               size_t(ND) * mps.max_faces * (VTM + 1) * es;
             const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer [threads][ncol]
             c->lds_mp = std::max(a1, a2);
@@ -2245,7 +2245,7 @@ bool setup_patches(meso_hip_ctx *c, int storage, bool keep_chunk, meso_patch::Pa
             ok = meso_patch::build_moment_patches<D>(mv, F, mps, perr);
             // [items | faces] | rows | tile velocities | values | f-bar+ rows (k_patch_moments)
             const size_t a1 = rec(IPWM, mps.max_items, mps.max_faces) + ((4 * size_t(mps.max_rows) + 31) & ~size_t(31)) +
-                              size_t(VTM) * 32 + size_t(ND) * mps.max_rows * VTM * es +
+                              mp_vtile_bytes(D, VTM) + size_t(ND) * mps.max_rows * VTM * es +
                               size_t(ND) * mps.max_faces * (VTM + 1) * es;
             const size_t a2 = size_t(NWP) * 64 * ncol * 8;  // reduction buffer [threads][ncol]
             c->lds_mp = std::max(a1, a2);
