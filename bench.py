@@ -234,12 +234,11 @@ def main():
     k32 = res["fp32"]["kernels"]
     dom = max(k32, key=lambda k: k32[k]["avg_ms"])
     dom_ms = k32[dom]["avg_ms"]
-    # DV-cells one launch of the dominant kernel processes: the interior cell
-    # kernel skips the cells with a boundary face (they have their own launch)
-    A = cs.arrays()
-    fc = A["face_cells"]
-    n_bcell = len(np.unique(fc[fc[:, 0] == fc[:, 1], 0]))
-    units = {"cell_update": (nc - n_bcell) * nv, "cell_update_bnd": n_bcell * nv}.get(dom, nc * nv)
+    # DV-cells one launch of the dominant kernel processes, as the library
+    # reports them: the update launch owns update_owned cells (every cell
+    # without a mirror boundary); the mirror-boundary cells have their own launch
+    owned = tiles["fp32"]["update_owned"]
+    units = {"cell_update": owned * nv, "cell_update_bnd": (nc - owned) * nv}.get(dom, nc * nv)
     achieved = b_alg * units / (dom_ms * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")

@@ -249,7 +249,9 @@ typedef struct meso_tile_config {
     int32_t vchunk;           /* velocities per block */
     int32_t update_vt;        /* velocities per tile, update kernel */
     int32_t moment_vt;        /* velocities per tile, moment kernel */
-    int32_t reserved;
+    int32_t update_owned;     /* cells whose update the dominant update launch owns
+                                 (patch kernels: every cell without a mirror boundary;
+                                 per-cell kernels: the cells without a boundary face) */
 } meso_tile_config;
 int meso_hip_tiles(const meso_hip_ctx *ctx, meso_tile_config *out);
 /* enable != 0: meso_hip_step and meso_hip_step_batch run steps in pairs as

@@ -3223,6 +3223,7 @@ int meso_hip_tiles(const meso_hip_ctx *c, meso_tile_config *out) {
     if (!c || !out) return MESO_E_INVALID;
     meso_tile_config t{};
     t.patches = c->patches ? 1 : 0;
+    t.update_owned = c->n_int;
     if (c->patches) {
         t.update_cells = c->up_maxo;
         t.moment_by_cells = c->mp_cells ? 1 : 0;

@@ -57,7 +57,7 @@ class StepParams(C.Structure):
 class TileConfig(C.Structure):
     _fields_ = [("patches", C.c_int32), ("update_cells", C.c_int32), ("moment_by_cells", C.c_int32),
                 ("moment_size", C.c_int32), ("vchunk", C.c_int32), ("update_vt", C.c_int32),
-                ("moment_vt", C.c_int32), ("reserved", C.c_int32)]
+                ("moment_vt", C.c_int32), ("update_owned", C.c_int32)]
 
 
 class StepReport(C.Structure):
