@@ -44,6 +44,9 @@ extern "C" {
 
 #define MESO_STORAGE_FP64 0 /* f, fluxes, gradients in FP64 */
 #define MESO_STORAGE_FP32 1 /* f, fluxes, gradients in FP32; moments, macros, residual in FP64 */
+/* f as FP32 plus an 8-bit extension of its mantissa (a 32-bit significand,
+ * 5 bytes per value); face values, moments and macros as for FP32 */
+#define MESO_STORAGE_FP32X 2
 
 #define MESO_LAYOUT_VELOCITY_MAJOR 0
 #define MESO_LAYOUT_CELL_MAJOR 1
@@ -142,9 +145,10 @@ typedef struct meso_step_params {
 /* Result of meso_hip_step(). */
 typedef struct meso_step_report {
     int32_t steps_done;
-    int32_t bad_cells;   /* cells with NaN density or T<0 / NaN T in the last step */
+    int32_t bad_cells;   /* unmasked cells with NaN density or T<0 / NaN T in the last step */
     int32_t first_bad_cell;
-    int32_t reserved;
+    int32_t masked_cells; /* bad cells of masked groups repaired in the steps run
+                             (cell_value_interp, cdugks_shakhov.cpp:766-789) */
 } meso_step_report;
 
 /* ======================================================================== */
@@ -168,6 +172,9 @@ int meso_case_reload_settings(meso_case *c);
 /* Per-cell initial density, velocity, temperature from the cell-group patches
  * (fixedValue or fromFile; cdugks_shakhov.cpp:108-131, cdugks.cpp:69-84). */
 int meso_case_initial_patch(const meso_case *c, double *rho, double *u3, double *T);
+/* mask[NC] = 1 for the cells whose group sets `mask switch True`
+ * (src/solver/boundary/patch.cpp:48-51, read at cdugks_shakhov.cpp:702,713). */
+int meso_case_cell_mask(const meso_case *c, int32_t *mask);
 /* Integer / string settings meso's time loop reads (max-step, write-interval, case-name, ...). */
 int meso_case_setting_int(const meso_case *c, const char *key, int32_t dflt, int32_t *out);
 int meso_case_setting_double(const meso_case *c, const char *key, double dflt, double *out);
@@ -217,9 +224,19 @@ int meso_hip_set_state(meso_hip_ctx *ctx, const double *f, int32_t layout, const
 int meso_hip_get_state(meso_hip_ctx *ctx, double *f, int32_t layout);
 
 /* Advance nsteps time steps. Stops early (returning MESO_E_NUMERIC) after a
- * step that produced NaN density or T<0 (cdugks_shakhov.cpp:698-719). */
+ * step that produced NaN density or T<0 in a cell outside the masked groups
+ * (cdugks_shakhov.cpp:698-719); that step is complete, as in meso. */
 int meso_hip_step(meso_hip_ctx *ctx, int32_t nsteps, meso_step_report *report);
 int meso_hip_set_params(meso_hip_ctx *ctx, const meso_step_params *params);
+/* Cells whose group sets `mask switch True` (mask[NC], 0/1; NULL clears). A
+ * Shakhov step repairs such a cell when its density is NaN or its T < 0, as
+ * CDUGKS_SHAKHOV::cell_value_interp does (cdugks_shakhov.cpp:698-719, 766-789;
+ * fvmMesh::interp_IDW, src/mesh/mesh_numerical.cpp:51-94): rho, u, T and q
+ * become the inverse-distance average of the face neighbours (cells before it
+ * in meso's loop order with their new values, the others with their old ones)
+ * and g, h the Shakhov equilibrium of those; q after the step is the cell's
+ * own heat-flux sum, as meso leaves it. Other models ignore the mask. */
+int meso_hip_set_mask(meso_hip_ctx *ctx, const int32_t *mask);
 
 /* Many independent cases at once (the batched cavity sweep): advances each of
  * the n contexts by nsteps, issuing step i of every context (each on its own

@@ -79,11 +79,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
         a.fm[5 * c] = cs.phi[0];
         for (int i = 0; i < D; ++i) a.fm[5 * c + 1 + i] = cs.phi[1 + i];
         if (MODEL == CDUGKS_SHAKHOV) a.fm[5 * c + 4] = cs.phi[D + 1];
-        if (!ok) {
-            atomicAdd(a.bad_count, 1);
-            atomicMin(a.first_bad, c);
-            atomicMin(a.bad_step, *a.step_ctr);
-        }
+        if (!ok) flag_bad<MODEL>(a, c);
         CellRec<S, D> r;
         if constexpr (sizeof(S) == 4) {
             r.en = to_eqf(cs.en);
@@ -118,14 +114,25 @@ __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
 
 // LSQ gradient g[0..D-1] of one item from the staged rows of distribution d:
 // sum_j a_j (f_j - f_c) (mesh_numerical.cpp:24-49).
+// A staged value in FP64. mid is 0, or (FP32X storage, whose staged words
+// lack their extension bytes) the midpoint bit of the missing byte, so that
+// the word reads without bias (x32_mid).
+template <class S>
+__device__ __forceinline__ double staged(S x, unsigned mid) {
+    if constexpr (sizeof(S) == 4)
+        return __longlong_as_double((long long)((unsigned long long)__double_as_longlong(double(x)) | mid));
+    else
+        return double(x);
+}
+
 template <class S, int VT, int D>
 __device__ __forceinline__ void item_grad(const meso_patch::ItemT<D> *it, const S *val_d, int vl, double fc,
-                                          double *g) {
+                                          double *g, unsigned mid) {
 #pragma unroll
     for (int k = 0; k < D; ++k) g[k] = 0.0;
 #pragma unroll
     for (int j = 0; j < meso_patch::Lim<D>::MAXN; ++j) {
-        const double df = double(val_d[it->nb[j] * VT + vl]) - fc;
+        const double df = staged(val_d[it->nb[j] * VT + vl], mid) - fc;
 #pragma unroll
         for (int k = 0; k < D; ++k) g[k] += it->a[j][k] * df;
     }
@@ -135,11 +142,11 @@ __device__ __forceinline__ void item_grad(const meso_patch::ItemT<D> *it, const 
 // staged rows (padding slots repeat the item); for the limiter.
 template <class S, int VT, int D>
 __device__ __forceinline__ void item_range(const meso_patch::ItemT<D> *it, const S *val_d, int vl, double fc,
-                                           double &mn, double &mx) {
+                                           double &mn, double &mx, unsigned mid) {
     mn = mx = fc;
 #pragma unroll
     for (int j = 0; j < meso_patch::Lim<D>::MAXN; ++j) {
-        const double w = double(val_d[it->nb[j] * VT + vl]);
+        const double w = staged(val_d[it->nb[j] * VT + vl], mid);
         mn = fmin(mn, w);
         mx = fmax(mx, w);
     }
@@ -439,6 +446,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     const int nvp = a.s.nvp;
     const double hdt = a.s.half_dt;
     const S *fin_g = (const S *)a.fin_g, *fin_h = (const S *)a.fin_h;
+    const unsigned mid = a.xin_g ? (1u << 28) : 0u;  // FP32X: read the words without bias
     // phase-C layout: thread = (face, velocity subset); FG faces per subset
     // (FG = the patch's faces, nvs = BLOCKP / FG subsets; the last
     // BLOCKP mod FG threads idle)
@@ -486,11 +494,11 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
             double fc[2], g[2][D], mn[2], mx[2];
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
-                fc[d] = double(val[(d * maxr + s) * VT + vl]);
+                fc[d] = staged(val[(d * maxr + s) * VT + vl], mid);
 #pragma unroll
                 for (int k = 0; k < D; ++k) g[d][k] = 0.0;
-                if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d]);
-                if constexpr (LIM) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d]);
+                if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d], mid);
+                if constexpr (LIM) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d], mid);
             }
             const double omega = LIM ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
             const double xv[3] = {X.x, X.y, X.z};
@@ -816,6 +824,14 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             // Shakhov: the cell's axis-table entries (tables of g_S^n and
             // g_S^{n+1}, in the storage type) are read first, so that their
             // latency overlaps the flux sums
+            unsigned char xe[2] = {0, 0};  // FP32X: extension bytes of f^n
+            if constexpr (sizeof(S) == 4) {
+                if (a.xin_g && L.ok) {
+                    const size_t o = size_t(c) * nvp + v;
+                    xe[0] = a.xin_g[o];
+                    if (SH) xe[1] = a.xin_h[o];
+                }
+            }
             S ta[2][3];
             if constexpr (TAB && SH) {
                 const S *tb = reinterpret_cast<const S *>(a.cell_tab) + size_t(c) * 2 * a.nt;
@@ -841,8 +857,14 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             }
             const CellRec<S, D> &R = crl[s];
             double qv[3] = {0.0, 0.0, 0.0};
-            const double fcg = double(val[s * VT + vl]);
-            const double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
+            double fcg = double(val[s * VT + vl]);
+            double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
+            if constexpr (sizeof(S) == 4) {
+                if (a.xin_g) {
+                    fcg = x32_decode(val[s * VT + vl], xe[0]);
+                    if (SH) fch = x32_decode(val[(maxr + s) * VT + vl], xe[1]);
+                }
+            }
             double og, oh = 0.0;
             if (SH) {
                 double gsn, hsn, gs, hs;
@@ -922,8 +944,27 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                 }
             }
             if (L.ok) {
-                fout_g[size_t(c) * nvp + v] = S(og);
-                if (SH) fout_h[size_t(c) * nvp + v] = S(oh);
+                const size_t o = size_t(c) * nvp + v;
+                bool done = false;
+                if constexpr (sizeof(S) == 4) {
+                    if (a.xout_g) {
+                        float wd;
+                        unsigned char e;
+                        x32_encode(og, wd, e);
+                        fout_g[o] = wd;
+                        a.xout_g[o] = e;
+                        if (SH) {
+                            x32_encode(oh, wd, e);
+                            fout_h[o] = wd;
+                            a.xout_h[o] = e;
+                        }
+                        done = true;
+                    }
+                }
+                if (!done) {
+                    fout_g[o] = S(og);
+                    if (SH) fout_h[o] = S(oh);
+                }
             } else {
 #pragma unroll
                 for (int i = 0; i < D; ++i) qv[i] = 0.0;

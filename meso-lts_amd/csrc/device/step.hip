@@ -170,7 +170,15 @@ __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restri
             return (1.0 - mix) * double(fbr[size_t(f) * nvp + v]) + mix * eq_full;
         }
     }
-    const double fs = double(fd[size_t(c) * nvp + v]);
+    // FP32X storage: every stored word with its extension byte
+    const unsigned char *xe = sizeof(T) == 4 ? (k == 0 ? a.xin_g : a.xin_h) : nullptr;
+    auto ld = [&](size_t i) -> double {
+        if constexpr (sizeof(T) == 4) {
+            if (xe) return x32_decode(float(fd[i]), xe[i]);
+        }
+        return double(fd[i]);
+    };
+    const double fs = ld(size_t(c) * nvp + v);
     double fb;
     const bool dead = !(sn >= VSMALL || sn < -VSMALL);
     if (!a.s.grad) {
@@ -178,7 +186,7 @@ __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restri
     } else {
         double gr[3] = {0.0, 0.0, 0.0};
         for (int j = a.nb_ptr[c]; j < a.nb_ptr[c + 1]; ++j) {
-            const double df = double(fd[size_t(a.nb_ids[j]) * nvp + v]) - fs;
+            const double df = ld(size_t(a.nb_ids[j]) * nvp + v) - fs;
             gr[0] += a.nb_a[3 * j] * df;
             gr[1] += a.nb_a[3 * j + 1] * df;
             gr[2] += a.nb_a[3 * j + 2] * df;
@@ -272,6 +280,21 @@ __device__ __forceinline__ void bnd_value_full(const KArgs &a, const BndParams &
 }
 
 // ================================================================ kernels
+// A bad cell (NaN density, T < 0 or NaN T) after the conservative update. A
+// Shakhov cell of a masked group is marked (mask 2) for k_mask_repair and
+// counted; any other stops the run after this step (cdugks_shakhov.cpp:698-719).
+template <int MODEL>
+__device__ __forceinline__ void flag_bad(const KArgs &a, int c) {
+    if (MODEL == CDUGKS_SHAKHOV && a.mask && a.mask[c]) {
+        const_cast<int *>(a.mask)[c] = 2;
+        atomicAdd(a.masked, 1);
+        return;
+    }
+    atomicAdd(a.bad_count, 1);
+    atomicMin(a.first_bad, c);
+    atomicMin(a.bad_step, *a.step_ctr);
+}
+
 // Sign of n0 . xi against the +-VSMALL dead-band (meso's upwind test,
 // cdugks_shakhov.cpp:227-254). FP64 storage evaluates it in FP64 exactly as
 // meso does; FP32 storage uses the FP32 product when it is clearly away from
@@ -1023,11 +1046,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
                 for (int i = 0; i < D; ++i) a.fm[5 * c + 1 + i] = cs.phi[1 + i];
                 if (SH) a.fm[5 * c + 4] = cs.phi[D + 1];
             }
-            if (!ok) {
-                atomicAdd(a.bad_count, 1);
-                atomicMin(a.first_bad, c);
-                atomicMin(a.bad_step, *a.step_ctr);
-            }
+            if (!ok) flag_bad<MODEL>(a, c);
         }
         nfc = a.cs_nf[c];
         if (TAB) {
@@ -1363,6 +1382,23 @@ __global__ __launch_bounds__(BLOCK) void k_dugks_tobp(KArgs a) {
 }
 
 // initial(): f from the patch values, macros from the discrete moments
+// Stores v at entry i of a distribution array: the FP32X word and its
+// extension byte when x is given, else the storage type.
+template <class T>
+__device__ __forceinline__ void st_val(T *p, unsigned char *x, size_t i, double v) {
+    if constexpr (sizeof(T) == 4) {
+        if (x) {
+            float w;
+            unsigned char e;
+            x32_encode(v, w, e);
+            p[i] = w;
+            x[i] = e;
+            return;
+        }
+    }
+    p[i] = T(v);
+}
+
 template <int MODEL, int D, class T, int LP>
 __global__ __launch_bounds__(BLOCK) void k_init_eq(KArgs a, const double *rp, const double *up, const double *Tp) {
     constexpr int IPB = BLOCK / LP;
@@ -1376,19 +1412,18 @@ __global__ __launch_bounds__(BLOCK) void k_init_eq(KArgs a, const double *rp, co
         const double zero[3] = {0, 0, 0};
         if (MODEL == CDUGKS_SHAKHOV) make_shakhov(e, rp[c], u, Tp[c], zero, a.s.R, a.s.Pr, D, a.s.K);
         else make_bgk(e, rp[c], u, a.s.RT);
-        T *og = (T *)a.fout_g + size_t(c) * a.s.nvp;
-        T *oh = (T *)a.fout_h + size_t(c) * a.s.nvp;
         for (int v = li; v < a.s.nv; v += LP) {
             const double X[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
             const double wv = a.w[v];
             double g, h = 0.0;
+            const size_t o = size_t(c) * a.s.nvp + v;
             if (MODEL == CDUGKS_SHAKHOV) {
                 shakhov_eq<double>(e, X, D, a.s.K, g, h);
-                oh[v] = T(h - (sizeof(T) == 4 ? a.fref_d[a.s.nv + v] : 0.0));
+                st_val((T *)a.fout_h, a.xout_h, o, h - (sizeof(T) == 4 ? a.fref_d[a.s.nv + v] : 0.0));
             } else {
                 g = bgk_eq<double>(e, X);
             }
-            og[v] = T(g - (sizeof(T) == 4 ? a.fref_d[v] : 0.0));
+            st_val((T *)a.fout_g, a.xout_g, o, g - (sizeof(T) == 4 ? a.fref_d[v] : 0.0));
             const double kk = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
             const double c0 = X[0] - u[0], c1 = X[1] - u[1], c2 = X[2] - u[2];
             const double cc = c0 * c0 + c1 * c1 + c2 * c2;
@@ -1433,18 +1468,18 @@ __global__ __launch_bounds__(BLOCK) void k_init_macros(KArgs a) {
     } else {
         make_bgk(e, a.rho_out[c], u, a.s.RT);
     }
-    T *og = (T *)a.fout_g + size_t(c) * a.s.nvp;
-    T *oh = (T *)a.fout_h + size_t(c) * a.s.nvp;
     for (int v = li; v < a.s.nv; v += LP) {
         const double X[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
         const double rg = sizeof(T) == 4 ? a.fref_d[v] : 0.0;
+        const size_t o = size_t(c) * a.s.nvp + v;
+        T *g0 = (T *)a.fout_g, *h0 = (T *)a.fout_h;
         if (MODEL == CDUGKS_SHAKHOV) {
             double g, h;
             shakhov_eq<double>(e, X, D, a.s.K, g, h);
-            og[v] = T(g - rg);
-            oh[v] = T(h - (sizeof(T) == 4 ? a.fref_d[a.s.nv + v] : 0.0));
+            st_val(g0, a.xout_g, o, g - rg);
+            st_val(h0, a.xout_h, o, h - (sizeof(T) == 4 ? a.fref_d[a.s.nv + v] : 0.0));
         } else {
-            og[v] = T(bgk_eq<double>(e, X) - rg);
+            st_val(g0, a.xout_g, o, bgk_eq<double>(e, X) - rg);
         }
     }
 }
@@ -1452,7 +1487,8 @@ __global__ __launch_bounds__(BLOCK) void k_init_macros(KArgs a) {
 // FP64 velocity-major [nv][nc] <-> device cell-major [nc][nvp] (32x32 tiles through LDS)
 template <class T, bool TO_DEVICE>
 __global__ __launch_bounds__(BLOCK) void k_transpose(const double *vm_in, double *vm_out, const T *cm_in, T *cm_out,
-                                                     int nc, int nv, int nvp, const double *fref, const int *slot_of) {
+                                                     int nc, int nv, int nvp, const double *fref, const int *slot_of,
+                                                     const unsigned char *x_in, unsigned char *x_out) {
     __shared__ double tile[32][33];
     const int c0 = blockIdx.x * 32, v0 = blockIdx.y * 32;
     const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
@@ -1466,14 +1502,23 @@ __global__ __launch_bounds__(BLOCK) void k_transpose(const double *vm_in, double
             int c = c0 + k, v = v0 + tx;
             if (c < nc && v < nv) {
                 const int sl = slot_of ? slot_of[v] : v;
-                cm_out[size_t(c) * nvp + sl] = T(tile[tx][k] - (fref ? fref[sl] : 0.0));
+                st_val(cm_out, x_out, size_t(c) * nvp + sl, tile[tx][k] - (fref ? fref[sl] : 0.0));
             }
         }
     } else {
         for (int k = ty; k < 32; k += 8) {
             int c = c0 + k, v = v0 + tx;
             const int sl = (v < nv && slot_of) ? slot_of[v] : v;
-            tile[tx][k] = (c < nc && v < nv) ? double(cm_in[size_t(c) * nvp + sl]) + (fref ? fref[sl] : 0.0) : 0.0;
+            double x = 0.0;
+            if (c < nc && v < nv) {
+                const size_t i = size_t(c) * nvp + sl;
+                x = double(cm_in[i]);
+                if constexpr (sizeof(T) == 4) {
+                    if (x_in) x = x32_decode(float(cm_in[i]), x_in[i]);
+                }
+                x += fref ? fref[sl] : 0.0;
+            }
+            tile[tx][k] = x;
         }
         __syncthreads();
         for (int k = ty; k < 32; k += 8) {
@@ -1515,10 +1560,79 @@ __global__ void k_residual_final(const double *part, int nblk, double *out) {
 
 __global__ void k_reset_flags(int *bad_count, int *first_bad, int *bad_step, int *halt, int *step_ctr, int steps) {
     *bad_count = 0;
+    step_ctr[1] = 0;  // masked cells repaired
     *first_bad = 0x7fffffff;
     *bad_step = 0x7fffffff;
     *halt = 0;
     *step_ctr = steps;
+}
+
+// Repair of the marked cells, after the step's update (one block, cells in
+// ascending order as meso's cell loop meets them; CDUGKS_SHAKHOV::
+// cell_value_interp, cdugks_shakhov.cpp:766-789, with fvmMesh::interp_IDW,
+// src/mesh/mesh_numerical.cpp:51-94): rho, u, T and q become the average of
+// the LSQ (face) neighbours weighted by 1/|dr|^2, where a neighbour before the
+// cell in that loop has its new values (and, if it was repaired, its repaired
+// q) and one after it its old values; g and h become the Shakhov equilibrium
+// of those. q_out keeps the cell's own heat-flux sum, as meso's q field is
+// overwritten by the sums after its loop. The repaired q is kept in q_in (the
+// old buffer, not read again in this step) for the later cells.
+template <int D, class T>
+__global__ __launch_bounds__(BLOCK) void k_mask_repair(KArgs a) {
+    if (*a.halt) return;
+    __shared__ int flag[BLOCK];
+    __shared__ double mac[8];  // rho, u[3], T, q[3]
+    for (int base = 0; base < a.s.nc; base += BLOCK) {
+        const int c0 = base + int(threadIdx.x);
+        flag[threadIdx.x] = (c0 < a.s.nc && a.mask[c0] == 2) ? 1 : 0;
+        if (__syncthreads_count(flag[threadIdx.x]) == 0) continue;
+        for (int t = 0; t < BLOCK; ++t) {
+            if (!flag[t]) continue;  // uniform: every thread reads the same entry
+            const int c = base + t;
+            if (threadIdx.x == 0) {
+                double sw = 0.0, sr = 0.0, sT = 0.0, su[3] = {0.0, 0.0, 0.0}, sq[3] = {0.0, 0.0, 0.0};
+                for (int j = a.nb_ptr[c]; j < a.nb_ptr[c + 1]; ++j) {
+                    const int n = a.nb_ids[j];
+                    const double w = a.nb_w[j];
+                    const bool upd = n < c;  // already met by meso's loop
+                    sw += w;
+                    sr += w * (upd ? a.rho_out[n] : a.rho_in[n]);
+                    sT += w * (upd ? a.T_out[n] : a.T_in[n]);
+                    for (int i = 0; i < 3; ++i) {
+                        su[i] += w * (upd ? a.u_out[3 * n + i] : a.u_in[3 * n + i]);
+                        sq[i] += w * a.q_in[3 * n + i];
+                    }
+                }
+                mac[0] = sr / sw;
+                mac[4] = sT / sw;
+                for (int i = 0; i < 3; ++i) {
+                    mac[1 + i] = su[i] / sw;
+                    mac[5 + i] = sq[i] / sw;
+                }
+                a.rho_out[c] = mac[0];
+                a.T_out[c] = mac[4];
+                for (int i = 0; i < 3; ++i) {
+                    a.u_out[3 * c + i] = mac[1 + i];
+                    const_cast<double *>(a.q_in)[3 * c + i] = mac[5 + i];
+                }
+                const_cast<int *>(a.mask)[c] = 1;
+            }
+            __syncthreads();
+            EqParams e;
+            const double u[3] = {mac[1], mac[2], mac[3]}, q[3] = {mac[5], mac[6], mac[7]};
+            make_shakhov(e, mac[0], u, mac[4], q, a.s.R, a.s.Pr, D, a.s.K);
+            for (int v = threadIdx.x; v < a.s.nv; v += BLOCK) {
+                const double X[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
+                double g, h;
+                shakhov_eq<double>(e, X, D, a.s.K, g, h);
+                const size_t o = size_t(c) * a.s.nvp + v;
+                const bool dev = sizeof(T) == 4;
+                st_val((T *)a.fout_g, a.xout_g, o, g - (dev ? a.fref_d[v] : 0.0));
+                st_val((T *)a.fout_h, a.xout_h, o, h - (dev ? a.fref_d[a.s.nv + v] : 0.0));
+            }
+            __syncthreads();
+        }
+    }
 }
 
 // Last launch of every step: the device-side step index (read by the kernels
@@ -1576,7 +1690,8 @@ struct meso_hip_ctx {
     int n_marks = 0;
     // device pointers
     int2 *face_cells = nullptr;
-    double *face_n = nullptr, *face_d = nullptr, *cf_sA = nullptr, *nb_a = nullptr, *cell_vol = nullptr;
+    double *face_n = nullptr, *face_d = nullptr, *cf_sA = nullptr, *nb_a = nullptr, *nb_w = nullptr,
+           *cell_vol = nullptr;
     float *face_df = nullptr, *nb_af = nullptr, *xif = nullptr;
     double *cell_S = nullptr;
     int *cs_nf = nullptr, *cs_face = nullptr, *cs_other = nullptr, *cs_own = nullptr, *cs_bnd = nullptr,
@@ -1631,6 +1746,14 @@ struct meso_hip_ctx {
     meso_mark_desc *marks = nullptr;
     double *xi = nullptr, *w = nullptr;
     void *f[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buffer][distribution]
+    // FP32X storage (storage is then MESO_STORAGE_FP32 with ext set): the
+    // extension bytes of f, [buffer][distribution]
+    bool ext = false;
+    unsigned char *fx[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    // masked cells (meso_hip_set_mask): 1 per cell of a group with `mask switch True`
+    int *mask = nullptr;
+    bool has_mask = false;
+    int masked_seen = 0;  // repaired cells counted up to the last report
     double *rho[2] = {nullptr, nullptr}, *u[2] = {nullptr, nullptr}, *T[2] = {nullptr, nullptr},
            *q[2] = {nullptr, nullptr};
     double *rho_res = nullptr, *u_res = nullptr, *T_res = nullptr, *q_res = nullptr;
@@ -1638,7 +1761,7 @@ struct meso_hip_ctx {
            *res_out = nullptr;
     EqParams *face_eq = nullptr;
     BndParams *bnd = nullptr;
-    int *flags = nullptr;  // bad_count, first_bad, bad_step, halt
+    int *flags = nullptr;  // bad_count, first_bad, bad_step, halt, step counter, masked count
     // reference equilibrium of the FP32 deviation storage
     EqParams ref{};
     double *fref_d = nullptr;
@@ -1747,6 +1870,7 @@ KArgs make_args(meso_hip_ctx *c) {
     a.nb_ptr = c->nb_ptr;
     a.nb_ids = c->nb_ids;
     a.nb_a = c->nb_a;
+    a.nb_w = c->nb_w;
     a.nb_af = c->nb_af;
     a.cell_vol = c->cell_vol;
     a.cell_S = c->cell_S;
@@ -1794,6 +1918,12 @@ KArgs make_args(meso_hip_ctx *c) {
     a.fin_h = c->f[i][1];
     a.fout_g = c->f[o][0];
     a.fout_h = c->f[o][1];
+    a.xin_g = c->fx[i][0];
+    a.xin_h = c->fx[i][1];
+    a.xout_g = c->fx[o][0];
+    a.xout_h = c->fx[o][1];
+    a.mask = c->has_mask ? c->mask : nullptr;
+    a.masked = c->flags + 5;
     a.fbar_g = c->fbar[0];
     a.fbar_h = c->fbar[1];
     a.rho_in = c->rho[i];
@@ -2008,6 +2138,7 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
         LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, D, T, TAB, true, LIM>),
                   ((c->n_bcell + WPB - 1) / WPB) * s.ntile_c, lds, a, c->n_int, c->n_bcell);
     if (MODEL == CDUGKS_SHAKHOV) LAUNCH(c, "cell_finish", (k_cell_finish<MODEL>), (s.nc + BLOCK - 1) / BLOCK, a);
+    if (MODEL == CDUGKS_SHAKHOV && c->has_mask) LAUNCH(c, "mask_repair", (k_mask_repair<D, T>), 1, a);
     HIP_TRY(c, hipGetLastError());
     return MESO_OK;
 }
@@ -2033,6 +2164,7 @@ int step_body(meso_hip_ctx *c, KArgs a) {
         LAUNCH_SH(c, "cell_update_bnd", (k_cell_update<MODEL, D, T, TAB, true, LIM>),
                   ((c->n_bcell + WPB - 1) / WPB) * s.ntile_c, lds, a, c->n_int, c->n_bcell);
     if (MODEL == CDUGKS_SHAKHOV) LAUNCH(c, "cell_finish", (k_cell_finish<MODEL>), (s.nc + BLOCK - 1) / BLOCK, a);
+    if (MODEL == CDUGKS_SHAKHOV && c->has_mask) LAUNCH(c, "mask_repair", (k_mask_repair<D, T>), 1, a);
     if (MODEL == DUGKS_BGK) {
         LAUNCH(c, "cell_finish", (k_cell_finish<MODEL>), (s.nc + BLOCK - 1) / BLOCK, a);
         LAUNCH(c, "dugks_tobp", (k_dugks_tobp<D, T, 64>), (s.nc + WPB - 1) / WPB, a);
@@ -2128,7 +2260,7 @@ void set_const(StepConst &s, const meso_step_params &p) {
 }
 
 int sync_flags(meso_hip_ctx *c) {
-    HIP_TRY(c, hipMemcpyAsync(c->flags_host, c->flags, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->flags_host, c->flags, 6 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MESO_OK;
 }
@@ -2137,6 +2269,7 @@ int reset_flags(meso_hip_ctx *c) {
     hipLaunchKernelGGL(k_reset_flags, dim3(1), dim3(1), 0, c->stream, c->flags, c->flags + 1, c->flags + 2,
                        c->flags + 3, c->flags + 4, c->steps_done);
     HIP_TRY(c, hipGetLastError());
+    c->masked_seen = 0;
     return MESO_OK;
 }
 
@@ -2343,8 +2476,12 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         return set_err(nullptr, MESO_E_INVALID, "unknown model");
     if (m->dim != 2 && m->dim != 3) return set_err(nullptr, MESO_E_INVALID, "dimension must be 2 or 3");
     if (p->dim != m->dim) return set_err(nullptr, MESO_E_INVALID, "params.dim differs from the mesh dimension");
-    if (storage != MESO_STORAGE_FP64 && storage != MESO_STORAGE_FP32)
-        return set_err(nullptr, MESO_E_INVALID, "storage must be MESO_STORAGE_FP64 or MESO_STORAGE_FP32");
+    if (storage != MESO_STORAGE_FP64 && storage != MESO_STORAGE_FP32 && storage != MESO_STORAGE_FP32X)
+        return set_err(nullptr, MESO_E_INVALID,
+                       "storage must be MESO_STORAGE_FP64, MESO_STORAGE_FP32 or MESO_STORAGE_FP32X");
+    // FP32X runs the FP32 kernels with the extension bytes beside the FP32 words
+    const bool ext = storage == MESO_STORAGE_FP32X;
+    if (ext) storage = MESO_STORAGE_FP32;
     if (m->n_cells <= 0 || m->n_faces <= 0 || d->n_vel <= 0) return set_err(nullptr, MESO_E_INVALID, "empty mesh or DVS");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -2358,6 +2495,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(nullptr, MESO_E_DEVICE, "hipStreamCreate failed"));
     c->storage = storage;
+    c->ext = ext;
     c->nd = p->model == MESO_MODEL_CDUGKS_SHAKHOV ? 2 : 1;
     set_const(c->s, *p);
     const int nc = m->n_cells, nf = m->n_faces, nv = d->n_vel, D = m->dim;
@@ -2729,6 +2867,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     UP(cf_sA, sA.data(), sA.size());
     UP(nb_ptr, m->nbr_ptr, nc + 1);
     UP(nb_ids, m->nbr_ids, m->n_nbr);
+    UP(nb_w, m->lsq_w, m->n_nbr);
     UP(nb_a, na.data(), na.size());
     UP(nb_af, naf.data(), naf.size());
     UP(cell_vol, m->cell_vol, nc);
@@ -2781,6 +2920,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             char *q = nullptr;
             if ((rc = dalloc(c, &q, fsz * es))) return fail(rc);
             c->f[b][k] = q;
+            if (ext && (rc = dalloc(c, &c->fx[b][k], fsz))) return fail(rc);
         }
     for (int b = 0; b < 2; ++b) {
         if ((rc = dalloc(c, &c->rho[b], nc)) || (rc = dalloc(c, &c->u[b], 3 * size_t(nc))) ||
@@ -2795,7 +2935,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         (rc = dalloc(c, &c->part_c, size_t(nc) * std::max(c->s.ntile_c, c->p_chunk_cap) * 4)) || (rc = dalloc(c, &c->fm, 5 * size_t(nc))) ||
         (rc = dalloc(c, &c->res_part, 8 * size_t((nc + BLOCK - 1) / BLOCK))) || (rc = dalloc(c, &c->res_out, 8)) ||
         (rc = dalloc(c, &c->face_eq, nf)) || (rc = dalloc(c, &c->face_eqf, nf)) || (rc = dalloc(c, &c->bnd, std::max(1, c->s.nb))) ||
-        (rc = dalloc(c, &c->flags, 5)) || (rc = dalloc(c, &c->fref_d, 2 * size_t(nv))) ||
+        (rc = dalloc(c, &c->flags, 6)) || (rc = dalloc(c, &c->mask, nc)) || (rc = dalloc(c, &c->fref_d, 2 * size_t(nv))) ||
         (rc = dalloc(c, &c->fref_f, 2 * size_t(nv))) || (rc = dalloc(c, &c->fref_d2, nv)) ||
         (rc = dalloc(c, &c->fref_f2, nv)))
         return fail(rc);
@@ -2824,7 +2964,11 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     }
     c->xi_h = xi_s;
     c->w_h = w_s;
-    if (hipHostMalloc((void **)&c->flags_host, 5 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    if (ext && (!c->patches || c->n_bcell > 0 || p->model == MESO_MODEL_DUGKS_BGK))
+        return fail(set_err(nullptr, MESO_E_INVALID,
+                            "MESO_STORAGE_FP32X needs the patch kernels: a CDUGKS model, at least 64 velocities "
+                            "and no symmetry or slip-wall boundary"));
+    if (hipHostMalloc((void **)&c->flags_host, 6 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(set_err(nullptr, MESO_E_DEVICE, "hipHostMalloc failed"));
     if ((rc = reset_flags(c))) return fail(rc);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_err(nullptr, MESO_E_DEVICE, "upload failed"));
@@ -2845,6 +2989,23 @@ void meso_hip_destroy(meso_hip_ctx *c) {
     if (c->flags_host) (void)hipHostFree(c->flags_host);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+int meso_hip_set_mask(meso_hip_ctx *c, const int32_t *mask) {
+    if (!c) return set_err(c, MESO_E_INVALID, "null context");
+    const int nc = c->s.nc;
+    std::vector<int> m(nc, 0);
+    bool any = false;
+    if (mask)
+        for (int i = 0; i < nc; ++i) {
+            m[i] = mask[i] ? 1 : 0;
+            any = any || m[i];
+        }
+    HIP_TRY(c, hipMemcpyAsync(c->mask, m.data(), size_t(nc) * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    c->has_mask = any && c->s.model == MESO_MODEL_CDUGKS_SHAKHOV;
+    ++c->kver;  // the launch sequence and the kernel arguments change
+    return MESO_OK;
 }
 
 int meso_hip_set_params(meso_hip_ctx *c, const meso_step_params *p) {
@@ -2934,25 +3095,26 @@ static int transpose(meso_hip_ctx *c, bool to_device, double *host, int which, i
     }
     dim3 grid((nc + 31) / 32, (nv + 31) / 32);
     void *dev = c->f[c->cur][which];
+    unsigned char *xdev = c->fx[c->cur][which];
     const bool f32 = c->storage == MESO_STORAGE_FP32;
     const double *fref = f32 ? c->fref_d + size_t(which) * nv : nullptr;
     if (to_device) {
         HIP_TRY(c, hipMemcpyAsync(stage, src, n * 8, hipMemcpyHostToDevice, c->stream));
         if (f32)
             hipLaunchKernelGGL((k_transpose<float, true>), grid, dim3(BLOCK), 0, c->stream, stage, nullptr, nullptr,
-                               (float *)dev, nc, nv, nvp, fref, c->slot_of);
+                               (float *)dev, nc, nv, nvp, fref, c->slot_of, nullptr, xdev);
         else
             hipLaunchKernelGGL((k_transpose<double, true>), grid, dim3(BLOCK), 0, c->stream, stage, nullptr, nullptr,
-                               (double *)dev, nc, nv, nvp, fref, c->slot_of);
+                               (double *)dev, nc, nv, nvp, fref, c->slot_of, nullptr, nullptr);
         HIP_TRY(c, hipGetLastError());
         HIP_TRY(c, hipStreamSynchronize(c->stream));
     } else {
         if (f32)
             hipLaunchKernelGGL((k_transpose<float, false>), grid, dim3(BLOCK), 0, c->stream, nullptr, stage,
-                               (const float *)dev, nullptr, nc, nv, nvp, fref, c->slot_of);
+                               (const float *)dev, nullptr, nc, nv, nvp, fref, c->slot_of, xdev, nullptr);
         else
             hipLaunchKernelGGL((k_transpose<double, false>), grid, dim3(BLOCK), 0, c->stream, nullptr, stage,
-                               (const double *)dev, nullptr, nc, nv, nvp, fref, c->slot_of);
+                               (const double *)dev, nullptr, nc, nv, nvp, fref, c->slot_of, nullptr, nullptr);
         HIP_TRY(c, hipGetLastError());
         double *dst = host + size_t(which) * n;
         if (layout == MESO_LAYOUT_CELL_MAJOR) {
@@ -3076,8 +3238,9 @@ static int step_finish(meso_hip_ctx *c, int32_t nsteps, meso_step_report *rep) {
         rep->steps_done = done;
         rep->bad_cells = bad;
         rep->first_bad_cell = bad > 0 ? c->flags_host[1] : -1;
-        rep->reserved = 0;
+        rep->masked_cells = c->flags_host[5] - c->masked_seen;
     }
+    c->masked_seen = c->flags_host[5];
     return bad > 0 ? set_err(c, MESO_E_NUMERIC, "NaN density or T<0 in a cell") : MESO_OK;
 }
 

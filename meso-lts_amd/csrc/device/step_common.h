@@ -31,6 +31,24 @@ enum Bc : int {
     BC_SYMMETRY = 8,
 };
 
+// FP32X storage: a value is an FP32 word plus one byte that extends its
+// significand by 8 bits. The FP64 value is rounded to nearest at 31 explicit
+// mantissa bits; the word holds the upper 23 of them (truncated, so the word
+// converted to FP64 has zeros below) and the byte the next 8. Decoding puts
+// the byte back into those bits of the FP64 pattern: one shift-or.
+__device__ __forceinline__ void x32_encode(double v, float &hi, unsigned char &ext) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v) + (1ull << 20);
+    ext = (unsigned char)((b >> 21) & 0xffu);
+    hi = float(__longlong_as_double((long long)(b & ~((1ull << 29) - 1))));  // exact
+}
+__device__ __forceinline__ double x32_decode(float hi, unsigned ext) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(double(hi));
+    return __longlong_as_double((long long)(b | ((unsigned long long)(ext & 0xffu) << 21)));
+}
+// The word alone, without bias: the midpoint of the values its byte can add
+// (the reconstruction reads FP32 words; the update reads word and byte).
+__device__ __forceinline__ double x32_mid(float hi) { return x32_decode(hi, 0x80u); }
+
 // Per-face equilibrium description, written by the face-macro kernel.
 // Shakhov: g_S = A exp(-B|c|^2) [1 + P1 (c.q)(|c|^2 iRT - D - 2)]
 //          h_S = g_M [hk + P2 (c.q)((|c|^2 iRT - D)(K+3-D) - 2K)],  c = xi - u

@@ -42,6 +42,7 @@ struct KArgs {
     const int *nb_ptr;          // [nc+1] LSQ neighbours
     const int *nb_ids;
     const double *nb_a;         // [nnz*3] w_j (Cx.dr_j, Cy.dr_j, Cz.dr_j)
+    const double *nb_w;         // [nnz] w_j = 1/|dr_j|^2 (inverse-distance weights of the mask repair)
     const float *nb_af;
     const double *cell_vol;
     const double *cell_S;       // [nc*3] sum over the cell's faces of +-A n (0 for a closed, planar-faced cell)
@@ -95,6 +96,10 @@ struct KArgs {
     // state
     const void *fin_g, *fin_h;  // cell-major, f^n
     void *fout_g, *fout_h;      // cell-major, f^{n+1}
+    // FP32X storage: the extension byte of every FP32 word of f^n / f^{n+1}
+    // (same indexing); null for FP32 and FP64 storage
+    const unsigned char *xin_g, *xin_h;
+    unsigned char *xout_g, *xout_h;
     const double *rho_in, *u_in, *T_in, *q_in;
     double *rho_out, *u_out, *T_out, *q_out;
     // scratch
@@ -110,6 +115,8 @@ struct KArgs {
     int *bad_step;
     int *halt;
     int *step_ctr;              // steps since the initial state (device counter)
+    const int *mask;            // [nc] 1: a bad cell is repaired (Shakhov); null: no masked group
+    int *masked;                // bad cells repaired so far (device counter)
     // LDS-staged patch kernels (2D): update patches (interior cells) and
     // moment patches (all faces); the velocity range is split into p_chunks
     // chunks of p_vchunk velocities

@@ -24,11 +24,17 @@ int fail(int code, const std::string &msg) {
 
 namespace meso_host {
 
-int model_from_solver(const std::string &s) {
+// meso's solver names (src/meso/main.cpp:47-57), also with the suffix "-hip"
+// under which a meso build registers the GPU solver class next to its own.
+int model_from_solver(const std::string &name) {
+    const std::string sfx = "-hip";
+    const std::string s = name.size() > sfx.size() && name.compare(name.size() - sfx.size(), sfx.size(), sfx) == 0
+                              ? name.substr(0, name.size() - sfx.size())
+                              : name;
     if (s == "cdugks@shakhov") return MESO_MODEL_CDUGKS_SHAKHOV;
     if (s == "cdugks@incompressible") return MESO_MODEL_CDUGKS_BGK;
     if (s == "dugks@incompressible") return MESO_MODEL_DUGKS_BGK;
-    throw std::invalid_argument("unsupported solver: " + s);
+    throw std::invalid_argument("unsupported solver: " + name);
 }
 
 void load_params(const Solver::Config &cfg, int model, int dim, double min_cell_size, double dvs_max_mag,
@@ -249,6 +255,10 @@ void Case::initial_patch(double *rho, double *u3, double *T) const {
     }
 }
 
+void Case::cell_mask(int32_t *mask) const {
+    for (auto &cell : mesh.cells) mask[cell.id] = config.get_cell_group(cell, mesh).patch.get_switch("mask") ? 1 : 0;
+}
+
 }  // namespace meso_host
 
 // ------------------------------------------------------------------ C ABI
@@ -305,6 +315,16 @@ int meso_case_initial_patch(const meso_case *c, double *rho, double *u3, double 
     if (!c) return fail(MESO_E_INVALID, "null argument");
     try {
         reinterpret_cast<const meso_host::Case *>(c)->initial_patch(rho, u3, T);
+        return MESO_OK;
+    } catch (const std::exception &e) {
+        return fail(MESO_E_INVALID, e.what());
+    }
+}
+
+int meso_case_cell_mask(const meso_case *c, int32_t *mask) {
+    if (!c || !mask) return fail(MESO_E_INVALID, "null argument");
+    try {
+        reinterpret_cast<const meso_host::Case *>(c)->cell_mask(mask);
         return MESO_OK;
     } catch (const std::exception &e) {
         return fail(MESO_E_INVALID, e.what());

@@ -33,6 +33,8 @@ struct Case {
     void mesh_desc(meso_mesh_desc &d) const;
     void dvs_desc(meso_dvs_desc &d) const;
     void initial_patch(double *rho, double *u3, double *T) const;
+    // 1 for the cells whose group sets `mask switch True` (src/solver/boundary/patch.cpp:48-51)
+    void cell_mask(int32_t *mask) const;
 };
 
 }  // namespace meso_host

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmeso_hip.so")
 MESO_OK = 0
 MESO_E_INVALID, MESO_E_IO, MESO_E_DEVICE, MESO_E_NUMERIC = -1, -2, -3, -4
 MODEL_CDUGKS_BGK, MODEL_CDUGKS_SHAKHOV, MODEL_DUGKS_BGK = 1, 2, 3
-STORAGE_FP64, STORAGE_FP32 = 0, 1
+STORAGE_FP64, STORAGE_FP32, STORAGE_FP32X = 0, 1, 2
 LAYOUT_VELOCITY_MAJOR, LAYOUT_CELL_MAJOR = 0, 1
 BC = dict(fluid_interior=0, farfield_inlet=1, pressure_inlet=2, freestream_inlet=3, farfield_outlet=4,
           pressure_outlet=5, wall=6, slip_wall=7, symmetry=8)
@@ -62,7 +62,7 @@ class TileConfig(C.Structure):
 
 class StepReport(C.Structure):
     _fields_ = [("steps_done", C.c_int32), ("bad_cells", C.c_int32), ("first_bad_cell", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("masked_cells", C.c_int32)]
 
 
 class MesoError(RuntimeError):
@@ -90,6 +90,7 @@ def lib() -> C.CDLL:
             "meso_case_params": ([vp, C.POINTER(StepParams)], C.c_int),
             "meso_case_reload_settings": ([vp], C.c_int),
             "meso_case_initial_patch": ([vp, _d, _d, _d], C.c_int),
+            "meso_case_cell_mask": ([vp, _i], C.c_int),
             "meso_case_setting_int": ([vp, C.c_char_p, C.c_int32, _i], C.c_int),
             "meso_case_setting_double": ([vp, C.c_char_p, C.c_double, _d], C.c_int),
             "meso_case_setting_str": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_int32], C.c_int),
@@ -110,6 +111,7 @@ def lib() -> C.CDLL:
             "meso_hip_set_graphs": ([vp, C.c_int32], C.c_int),
             "meso_hip_tiles": ([vp, C.POINTER(TileConfig)], C.c_int),
             "meso_hip_set_params": ([vp, C.POINTER(StepParams)], C.c_int),
+            "meso_hip_set_mask": ([vp, _i], C.c_int),
             "meso_hip_get_macros": ([vp, _d, _d, _d, _d], C.c_int),
             "meso_hip_get_flux_moments": ([vp, _d, _d, _d], C.c_int),
             "meso_hip_residual": ([vp, _d], C.c_int),
@@ -201,6 +203,12 @@ class Case:
         self._check(lib().meso_case_initial_patch(self._h, dptr(rho), dptr(u), dptr(T)))
         return rho, u.reshape(nc, 3), T
 
+    def cell_mask(self):
+        """1 for the cells of groups with `mask switch True` (meso_case_cell_mask)."""
+        m = np.zeros(self.n_cells, dtype=np.int32)
+        self._check(lib().meso_case_cell_mask(self._h, iptr(m)))
+        return m
+
     def write_output(self, root, step, solution_time, rho, u, T=None, q=None, flux=None):
         """The files meso's output() writes for `step` (meso_case_write_output):
         rho (NC), u (NC x 3); Shakhov also T, q (NC x 3) and flux = (m0, m1, m2)."""
@@ -288,6 +296,14 @@ class Device:
     def init_from_macros(self, rho, u, T, q):
         self._check(lib().meso_hip_init_from_macros(self._h, dptr(as_f64(rho)), dptr(as_f64(u)), dptr(as_f64(T)),
                                                     dptr(as_f64(q))))
+
+    def set_mask(self, mask=None):
+        """Cells repaired instead of stopping the run (meso_hip_set_mask); None
+        takes the case's `mask switch True` groups."""
+        if mask is None:
+            mask = self.case.cell_mask()
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        self._check(lib().meso_hip_set_mask(self._h, iptr(m)))
 
     def set_state(self, f, rho, u, T=None, q=None, layout=LAYOUT_VELOCITY_MAJOR):
         self._check(lib().meso_hip_set_state(self._h, dptr(as_f64(f)), layout, dptr(as_f64(rho)), dptr(as_f64(u)),

@@ -290,10 +290,12 @@ def fmt(x: float) -> str:
 
 def shakhov_cavity_case(case_dir: str, mesh: GambitMesh, dvs: tuple, mu0: float, lid_u: float, dt: float | None = None,
                         cfl: float = 0.5, Pr: float = 2.0 / 3.0, max_step: int = 1000, residual_limit: float = 1e-7,
-                        name: str = "cavity", omega: float = 0.5, T_wall: float = 1.0, limiter: bool = False) -> str:
+                        name: str = "cavity", omega: float = 0.5, T_wall: float = 1.0, limiter: bool = False,
+                        mask: bool = False) -> str:
     """Write mesh, DVS and a cdugks@shakhov case into case_dir; returns the case path.
     dvs = ("nc", degree, mount, scale) or ("sparse-gh", n, RT). dt defaults to
-    cfl * min_cell_size / max|xi|."""
+    cfl * min_cell_size / max|xi|. mask: the fluid group sets `mask switch True`
+    (bad cells are repaired instead of stopping the run)."""
     os.makedirs(case_dir, exist_ok=True)
     write_neu(mesh, os.path.join(case_dir, "mesh.neu"))
     if dvs[0] == "nc":
@@ -317,6 +319,8 @@ def shakhov_cavity_case(case_dir: str, mesh: GambitMesh, dvs: tuple, mu0: float,
         "ref-viscosity": fmt(mu0), "vhs-index": fmt(omega),
     }
     groups = [{"name": "fluid", "density": "fixedValue scalar 1.0", "temperature": "fixedValue scalar 1.0"}]
+    if mask:
+        groups[0]["mask"] = "switch True"
     zero = "0.0 0.0 0.0"
     marks = [
         {"name": "lid", "type": "wall", "velocity": f"fixedValue vector {fmt(lid_u)} 0.0 0.0",

@@ -22,6 +22,7 @@
 //   residual (A8)          hand example
 #include "meso_oracle.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -92,6 +93,8 @@ struct oracle_ctx {
     std::vector<double> fm0, fm2;
     std::vector<V3> fm1;
     std::vector<double> rho_res, T_res;
+    std::vector<int> mask;  // 1: the cell's group sets `mask switch True`
+    int masked_cells = 0;   // repaired cells, accumulated over oracle_step calls
     std::vector<V3> u_res, q_res;
     int bad_cells = 0, first_bad = -1;
 
@@ -407,8 +410,54 @@ struct oracle_ctx {
         }
     }
 
+    // ---------------- mask repair (CDUGKS_SHAKHOV::cell_value_interp,
+    // cdugks_shakhov.cpp:766-789, called from the monitor at :698-719, with
+    // fvmMesh::interp_IDW, src/mesh/mesh_numerical.cpp:51-94). meso meets the
+    // cells in ascending order inside its update loop, so a neighbour before
+    // the cell has its new (or repaired) macros and one after it still has the
+    // old ones; q_cell holds q^n until the loop ends (repaired cells set theirs
+    // to the average), and after the loop every q is the heat-flux sum m3, the
+    // repaired cells' included. Rows are the oracle's velocity-major arrays.
+    void mask_repair(int c, const std::vector<double> &rho_o, const std::vector<V3> &u_o,
+                     const std::vector<double> &T_o, std::vector<V3> &q_loop) {
+        double sw = 0.0, sr = 0.0, sT = 0.0;
+        V3 su, sq;
+        for (int k = nb_ptr[c]; k < nb_ptr[c + 1]; ++k) {
+            const int n = nb_ids[k];
+            const double wi = 1.0 / (dr[k] * dr[k]);
+            const bool upd = n < c;
+            sw += wi;
+            sr += wi * (upd ? rho[n] : rho_o[n]);
+            sT += wi * (upd ? T[n] : T_o[n]);
+            su += wi * (upd ? u[n] : u_o[n]);
+            sq += wi * q_loop[n];
+        }
+        const double r = sr / sw, t = sT / sw;
+        const V3 uu = su / sw, qq = sq / sw;
+        rho[c] = r;
+        u[c] = uu;
+        T[c] = t;
+        q_loop[c] = qq;
+        for (int p = 0; p < NV; ++p) {
+            const V3 cv = xi[p] - uu;
+            const double cc = cv * cv, cq = cv * qq;
+            const double gm = g_maxwell(r, t, cc);
+            G(p, c) = g_shakhov(r, t, cc, cq, gm);
+            H(p, c) = h_shakhov(r, t, cc, cq, gm);
+        }
+    }
+
     // ---------------- Shakhov fvm_update (cdugks_shakhov.cpp:610-722)
     void fvm_update_shakhov() {
+        const bool any_mask = std::find(mask.begin(), mask.end(), 1) != mask.end();
+        std::vector<double> rho_o, T_o;
+        std::vector<V3> u_o, q_loop;
+        if (any_mask) {
+            rho_o = rho;
+            T_o = T;
+            u_o = u;
+            q_loop = q;
+        }
 #pragma omp parallel for schedule(static)
         for (int c = 0; c < NC; ++c) {
             double m0 = 0, m2 = 0;
@@ -476,12 +525,19 @@ struct oracle_ctx {
             q[c] = m3;
             if (std::isnan(r) || t < 0.0 || std::isnan(t)) ++bad;
         }
-        if (bad)
-            for (int c = 0; c < NC; ++c)
-                if (std::isnan(rho[c]) || T[c] < 0.0 || std::isnan(T[c])) {
-                    first = c;
-                    break;
+        if (bad) {
+            bad = 0;
+            for (int c = 0; c < NC; ++c) {
+                if (!(std::isnan(rho[c]) || T[c] < 0.0 || std::isnan(T[c]))) continue;
+                if (any_mask && mask[c]) {
+                    mask_repair(c, rho_o, u_o, T_o, q_loop);
+                    ++masked_cells;
+                    continue;
                 }
+                if (first < 0) first = c;
+                ++bad;
+            }
+        }
         bad_cells = bad;
         first_bad = first;
     }
@@ -817,6 +873,14 @@ int oracle_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_st
 
 void oracle_destroy(oracle_ctx *o) { delete o; }
 
+int oracle_set_mask(oracle_ctx *o, const int32_t *m) {
+    if (!o) return fail(MESO_E_INVALID, "null argument");
+    o->mask.assign(o->NC, 0);
+    if (m)
+        for (int c = 0; c < o->NC; ++c) o->mask[c] = m[c] ? 1 : 0;
+    return MESO_OK;
+}
+
 int oracle_set_params(oracle_ctx *o, const meso_step_params *p) {
     if (!o || !p) return fail(MESO_E_INVALID, "null argument");
     o->set_params(*p);
@@ -871,6 +935,7 @@ int oracle_get_state(oracle_ctx *o, double *f, int32_t layout) {
 
 int oracle_step(oracle_ctx *o, int32_t n, meso_step_report *rep) {
     if (!o || n < 0) return fail(MESO_E_INVALID, "bad argument");
+    const int masked0 = o->masked_cells;
     int done = 0;
     for (; done < n; ++done) {
         o->do_step();
@@ -883,7 +948,7 @@ int oracle_step(oracle_ctx *o, int32_t n, meso_step_report *rep) {
         rep->steps_done = done;
         rep->bad_cells = o->bad_cells;
         rep->first_bad_cell = o->first_bad;
-        rep->reserved = 0;
+        rep->masked_cells = o->masked_cells - masked0;
     }
     return o->bad_cells ? fail(MESO_E_NUMERIC, "NaN density or T<0 in a cell") : MESO_OK;
 }

@@ -29,6 +29,8 @@ void oracle_set_threads(int32_t n);
 int32_t oracle_get_threads(void);
 
 int oracle_set_params(oracle_ctx *ctx, const meso_step_params *params);
+/* cells of groups with `mask switch True` (NULL clears); Shakhov repairs them */
+int oracle_set_mask(oracle_ctx *ctx, const int32_t *mask);
 int oracle_init_equilibrium(oracle_ctx *ctx, const double *rho, const double *u3, const double *T);
 int oracle_init_from_macros(oracle_ctx *ctx, const double *rho, const double *u3, const double *T,
                             const double *q3);

@@ -36,6 +36,7 @@ def lib():
             "oracle_set_threads": ([C.c_int32], None),
             "oracle_get_threads": ([], C.c_int32),
             "oracle_set_params": ([vp, C.POINTER(StepParams)], C.c_int),
+            "oracle_set_mask": ([vp, C.POINTER(C.c_int32)], C.c_int),
             "oracle_init_equilibrium": ([vp, _d, _d, _d], C.c_int),
             "oracle_init_from_macros": ([vp, _d, _d, _d, _d], C.c_int),
             "oracle_set_state": ([vp, _d, C.c_int32, _d, _d, _d, _d], C.c_int),
@@ -94,6 +95,14 @@ class Oracle:
 
     def set_params(self, p):
         self._check(lib().oracle_set_params(self._h, C.byref(p)))
+
+    def set_mask(self, mask=None):
+        """Cells repaired instead of stopping (oracle_set_mask); None takes the
+        case's `mask switch True` groups."""
+        if mask is None:
+            mask = self.case.cell_mask()
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        self._check(lib().oracle_set_mask(self._h, m.ctypes.data_as(C.POINTER(C.c_int32))))
 
     def init_equilibrium(self, rho=None, u=None, T=None):
         if rho is None:

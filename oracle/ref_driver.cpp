@@ -125,6 +125,35 @@ void dump_macros(Solver::CDUGKS &s, int step, bool state) { dump_bgk(s, step, st
 template <>
 void dump_macros(Solver::DUGKS &s, int step, bool state) { dump_bgk(s, step, state); }
 
+// The residuals meso's do_step() forms at a check step (every
+// Solver::residual_interval steps, cdugks_shakhov.cpp:730-744, cdugks.cpp:343-345,
+// dugks.cpp:352-354), recomputed with meso's own Solver::residual() on copies of
+// the fields it compares: rho, u (3), T, q (3) (BGK: T and q are 0).
+struct ResBase {
+    std::vector<Field<Scalar>> s;
+    std::vector<Field<Vector>> v;
+};
+ResBase res_base(Solver::CDUGKS_SHAKHOV &x) { return {{x.rho_cell_res, x.T_cell_res}, {x.vel_cell_res, x.q_cell_res}}; }
+template <class S>
+ResBase res_base(S &x) { return {{x.rho_cell_res}, {x.vel_cell_res}}; }
+void dump_residual(Solver::CDUGKS_SHAKHOV &x, ResBase &b, int step) {
+    double r[8];
+    r[0] = Solver::residual(b.s[0], x.rho_cell);
+    Vector u = Solver::residual(b.v[0], x.vel_cell);
+    r[4] = Solver::residual(b.s[1], x.T_cell);
+    Vector q = Solver::residual(b.v[1], x.q_cell);
+    r[1] = u.x, r[2] = u.y, r[3] = u.z, r[5] = q.x, r[6] = q.y, r[7] = q.z;
+    put("step" + std::to_string(step) + "_residual.f64", r, sizeof(r));
+}
+template <class S>
+void dump_residual(S &x, ResBase &b, int step) {
+    double r[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    r[0] = Solver::residual(b.s[0], x.rho_cell);
+    Vector u = Solver::residual(b.v[0], x.vel_cell);
+    r[1] = u.x, r[2] = u.y, r[3] = u.z;
+    put("step" + std::to_string(step) + "_residual.f64", r, sizeof(r));
+}
+
 // write: also call the solver's own output() at the dumped steps (files under
 // the working directory, as meso writes them)
 template <class S>
@@ -142,7 +171,11 @@ int run(ArgParser &parser, Solver::Config &config, const std::set<int> &steps, b
     }
     int last = steps.empty() ? 0 : *steps.rbegin();
     for (int i = 1; i <= last; ++i) {
+        const bool check = i % Solver::residual_interval == 0;
+        ResBase base;
+        if (check) base = res_base(solver);
         solver.do_step();
+        if (check) dump_residual(solver, base, i);
         if (steps.count(i)) {
             dump_macros(solver, i, state);
             if (write) solver.output();

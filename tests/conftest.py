@@ -23,8 +23,16 @@ def rel_l2(a, b):
 
 
 def max_rel(a, b):
+    """Largest |a - b| relative to max |b|. NaN entries must sit in the same
+    places in both (meso's mask repair leaves NaN heat fluxes); inf otherwise."""
     a = np.asarray(a, dtype=np.float64).ravel()
     b = np.asarray(b, dtype=np.float64).ravel()
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return np.inf
+    a, b = a[~nb], b[~nb]
+    if a.size == 0:
+        return 0.0
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 

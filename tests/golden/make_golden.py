@@ -17,6 +17,13 @@ files/nc.dvs and the settings of files/case-re400.txt / files/case-kn0.075.txt,
 and neumann, which is meso's files/case-neumann.txt with its two meshes
 (files/neumann/: the physical mesh and the unstructured velocity mesh).
 
+Where a run reaches meso's residual check steps (every 100 steps), golden.npz
+also holds stepN_residual: the 8 residuals (rho, u, v, w, T, qx, qy, qz; BGK
+rho, u, v, w and zeros) that meso's do_step() forms there, recomputed by the
+driver with meso's own Solver::residual() on copies of the compared fields.
+shak_tri6_mask is shak_tri6 with `mask switch True` on its group: meso
+repairs the cell that goes bad at step 26 (cell_value_interp) and runs on.
+
 The out_* fixtures hold the files meso's own output() wrote (case-name/
 grid.plt, step-N/data.plt, latest/ and step-N/ text fields; DUGKS result
 files and np-data/) after steps 0 and 2, for the output writer's tests.
@@ -54,6 +61,13 @@ def gen_inputs(name, d, ref_root):
         return mg.shakhov_cavity_case(d, mg.quad_cavity(6, jitter=0.1, seed=3), ("nc", 1, 7, 4.5), mu075, 0.2)
     if name == "shak_tri6":
         return mg.shakhov_cavity_case(d, mg.tri_cavity(6, jitter=0.2, seed=1), ("nc", 2, 4, 4.5), mu075, 0.2)
+    if name == "shak_tri6_mask":
+        # shak_tri6 goes bad (NaN density in cell 10) at step 26; here its group
+        # sets `mask switch True`, so meso repairs the cell and runs on
+        return mg.shakhov_cavity_case(d, mg.tri_cavity(6, jitter=0.2, seed=1), ("nc", 2, 4, 4.5), mu075, 0.2,
+                                      mask=True)
+    if name == "shak_quad6_res":
+        return mg.shakhov_cavity_case(d, mg.quad_cavity(6, jitter=0.1, seed=3), ("nc", 1, 7, 4.5), mu075, 0.2)
     if name == "shak_hex3":
         return mg.shakhov_cavity_case(d, mg.hex_cavity(3, jitter=0.05, seed=2), ("sparse-gh", 4, 0.5),
                                       mg.viscosity_from_kn(0.1), 0.1)
@@ -126,6 +140,8 @@ CASES = {
     "bgk_tri6": ([1, 10], None),
     "shak_quad6": ([1, 2, 10, 50], 2),
     "shak_tri6": ([1, 10], None),
+    "shak_tri6_mask": ([1, 25, 26, 27, 28], 26),
+    "shak_quad6_res": ([100, 200], None),
     "shak_hex3": ([1, 5], None),
     "shak_quad6_lim": ([1, 2, 10, 50], None),
     "shak_tri6_lim": ([1, 10], None),
@@ -209,6 +225,11 @@ def run_ref(case, steps, state_step):
                 p = os.path.join(out, f"step{s}_{k}.f64")
                 if os.path.exists(p):
                     g[f"step{s}_{k}"] = np.fromfile(p, dtype=np.float64).reshape(shape)
+        # meso's residuals at its check steps (every 100 steps)
+        for s in range(100, max(steps) + 1, 100):
+            p = os.path.join(out, f"step{s}_residual.f64")
+            if os.path.exists(p):
+                g[f"step{s}_residual"] = np.fromfile(p, dtype=np.float64)
         if state_step:
             for k in ("f", "g", "h"):
                 p = os.path.join(out, f"step{state_step}_{k}.f64")
