@@ -16,6 +16,7 @@ OpenMP on this host, one step of the same workload).
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scale s]
                        [--storage both|fp32|fp64] [--copy-calibration MIB]
                        [--autotune] [--graphs] [--batch N]
+       python bench.py --converge [--workload W] [--scale s] [--max-steps N] [--time-limit S] [--run-on]
 
 With --autotune each context first runs meso_hip_autotune (trial steps of the
 initial state, not committed) to choose its patch and velocity-chunk tiles.
@@ -37,6 +38,15 @@ from meso_hip import capi, configs  # noqa: E402
 
 METRIC = "DV-cell updates/s, HBM GB/s % of peak; FP32-store vs FP64 at <=1e-6 rel err"
 HBM_PEAK_GBS = 8000.0
+# Speed of the restatement (the oracle, OpenMP threads) over meso itself (MPI
+# ranks over the velocity set, src/solver/dvs/dvs_mpi.cpp:5-28), compiled from
+# its sources with -O2 (oracle/Makefile.ref), timed on identical inputs in the
+# 8-core build container: the Kn = 1 cavity at half resolution (100^2 quads x
+# 101^2 velocities), seconds per step from 1- and 3-step runs.
+CALIBRATION = {"oracle_over_reference_1core": 1.91, "oracle_over_reference_8cores": 1.06,
+               "reference_8ranks_dv_cell_per_s": 7.73e6, "oracle_8threads_dv_cell_per_s": 8.21e6,
+               "where": "8-core Intel Xeon build container, not the GPU host",
+               "workload": "cavity_kn1_quad at scale 0.5 (10,000 cells x 10,201 velocities)"}
 WORKLOAD = "cavity_kn1_quad"
 
 
@@ -81,8 +91,10 @@ def cpu_baseline(case_path, steps):
     dt = time.perf_counter() - t0
     ups = cs.n_cells * cs.n_vel * steps / dt
     return {"value": ups, "unit": "DV-cell updates/s", "cores": get_threads(), "kind": "port",
+            "label": "restatement",
             "sample": f"{steps} full step(s) of the same workload ({cs.n_cells} cells x {cs.n_vel} velocities), "
-                      f"FP64 oracle with OpenMP, {dt:.1f} s"}
+                      f"FP64 oracle with OpenMP, {dt:.1f} s",
+            "calibration": CALIBRATION}
 
 
 def copy_calibration(mib):
@@ -153,6 +165,97 @@ def batch_main(args):
     print(json.dumps(line), flush=True)
 
 
+def macro_errors(a, b, dim, shakhov):
+    """Relative L2 over cells (unweighted) of every macro component of a against b."""
+    e = {"rho": rel_l2(a["rho"], b["rho"])}
+    for i, c in enumerate("UVW"[:dim]):
+        e[c] = rel_l2(a["u"][:, i], b["u"][:, i])
+    if shakhov:
+        e["T"] = rel_l2(a["T"], b["T"])
+        for i, c in enumerate("xyz"[:dim]):
+            e["q" + c] = rel_l2(a["q"][:, i], b["q"][:, i])
+    return e
+
+
+def converge_main(args):
+    """The workload run to meso's stopping rule in FP64, FP32 and FP32X storage
+    side by side: every 100 steps meso's residuals (meso_hip_residual), converged
+    after 10 consecutive checks with every residual <= the case's
+    residual-limit (cdugks_shakhov.cpp:730-756). Reports the step each storage
+    converged at and every macro's FP32 / FP32X error against the FP64 run at
+    the FP64 convergence step (and at the end of the run). The run ends when
+    FP64 has converged (with --run-on: when every storage has), or at
+    --max-steps / --time-limit."""
+    work = os.path.join(tempfile.gettempdir(), "meso_hip_bench")
+    cs = capi.Case(configs.build(args.workload, work, args.scale))
+    dim = cs.mesh.dim
+    shakhov = cs.model == capi.MODEL_CDUGKS_SHAKHOV
+    limit = cs.setting_double("residual-limit", 1e-6)
+    if args.residual_limit > 0:
+        limit = args.residual_limit
+    idx = ([0, 4, 1, 2, 5, 6] if dim == 2 else [0, 4, 1, 2, 3, 5, 6, 7]) if shakhov else list(range(1 + dim))
+    kinds = [("fp64", capi.STORAGE_FP64), ("fp32", capi.STORAGE_FP32), ("fp32x", capi.STORAGE_FP32X)]
+    devs = {}
+    for tag, st in kinds:
+        try:
+            d = capi.Device(cs, st)
+        except capi.MesoError as e:
+            log(f"[converge] {tag}: {e}")
+            continue
+        d.init_equilibrium()
+        d.residual()
+        devs[tag] = d
+    log(f"[converge] {args.workload} scale {args.scale}: NC={cs.n_cells} NV={cs.n_vel} dt={cs.params.dt:.4e} "
+        f"residual-limit {limit:g}")
+    state = {t: 0 for t in devs}
+    conv_at = {t: None for t in devs}
+    history = {t: [] for t in devs}
+    at_fp64 = None
+    curve = []
+    t0 = time.time()
+    step = 0
+    while step < args.max_steps and time.time() - t0 < args.time_limit:
+        live = [t for t in devs if conv_at[t] is None]
+        if not live:
+            break
+        for t in live:
+            rep = devs[t].step(100)
+            if rep.bad_cells:
+                raise RuntimeError(f"{t}: numerical failure at step {step + rep.steps_done}")
+        step += 100
+        for t in live:
+            r = np.asarray(devs[t].residual())[idx]
+            rmax = float(np.max(r))
+            history[t].append(rmax)
+            state[t] = state[t] + 1 if bool(np.all(r <= limit)) else 0
+            if state[t] >= 10:
+                conv_at[t] = step
+        errs = None
+        if step % args.curve_every == 0 or conv_at["fp64"] == step:
+            ref = devs["fp64"].macros()
+            errs = {t: macro_errors(devs[t].macros(), ref, dim, shakhov) for t in devs if t != "fp64"}
+            curve.append({"step": step, **{t: max(e.values()) for t, e in errs.items()}})
+        if conv_at["fp64"] == step:
+            at_fp64 = {"step": step, "errors": errs}
+        log(f"[converge] step {step} " + " ".join(f"{t} res {history[t][-1]:.2e}({state[t]})" for t in live) +
+            ("" if errs is None else " | err " + " ".join(f"{t} {max(e.values()):.2e}" for t, e in errs.items())) +
+            f" t={time.time() - t0:.0f}s")
+        if conv_at["fp64"] is not None and not args.run_on:
+            break
+    ref = devs["fp64"].macros()
+    final = {t: macro_errors(devs[t].macros(), ref, dim, shakhov) for t in devs if t != "fp64"}
+    line = {
+        "metric": "FP32-store vs FP64 rel L2 at convergence", "workload": args.workload, "mesh_scale": args.scale,
+        "cells": cs.n_cells, "velocities": cs.n_vel, "dt": cs.params.dt, "residual_limit": limit,
+        "rule": "10 consecutive checks every 100 steps, every residual <= limit (cdugks_shakhov.cpp:730-756)",
+        "converged_at": conv_at, "steps_run": step, "seconds": time.time() - t0,
+        "at_fp64_convergence": at_fp64, "at_end": {"step": step, "errors": final},
+        "max_error_curve": curve,
+        "residual_history_every_100": {t: h[-50:] for t, h in history.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,7 +275,19 @@ def main():
                     help="run meso_hip_autotune before timing (off by default: the default tiles are "
                          "deterministic, so that profiles of the same command see the same kernels)")
     ap.add_argument("--graphs", action="store_true", help="replay captured step pairs (meso_hip_set_graphs)")
+    ap.add_argument("--converge", action="store_true",
+                    help="instead: run FP64, FP32 and FP32X storage to meso's convergence rule and report the "
+                         "FP32 / FP32X errors against FP64 there")
+    ap.add_argument("--max-steps", type=int, default=200000, help="--converge: step cap")
+    ap.add_argument("--time-limit", type=float, default=900.0, help="--converge: wall-time cap in seconds")
+    ap.add_argument("--residual-limit", type=float, default=0.0, help="--converge: override the case's limit")
+    ap.add_argument("--curve-every", type=int, default=1000, help="--converge: steps between error samples")
+    ap.add_argument("--run-on", action="store_true",
+                    help="--converge: keep stepping FP32 / FP32X after FP64 converged until they do")
     args = ap.parse_args()
+    if args.converge:
+        converge_main(args)
+        return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return

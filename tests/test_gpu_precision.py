@@ -1,0 +1,125 @@
+"""FP32X storage (an FP32 word plus an 8-bit extension of its significand per
+stored value) on the GPU against the FP64 oracle, and its state transfers.
+
+Tolerance: relative L2 <= 1e-7 on every macro component after 200 steps,
+ten times below the 1e-6 the FP32-storage precision target allows. An FP64 CPU
+restatement that rounds the stored values to 31 mantissa bits, the face values
+to FP32 and reads the stored words for the reconstruction (the FP32X step)
+stays below 1e-7 on a 20x20 Kn = 1 cavity with 21^2 velocities through its
+whole run to a steady state, where FP32 storage with round-to-nearest
+stagnates at 1e-4.
+"""
+import numpy as np
+import pytest
+
+from conftest import max_rel, rel_l2
+from meso_hip import capi
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FP32X_TOL = 1e-7
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def macro_errors(cs, md, mo):
+    out = {"rho": rel_l2(md["rho"], mo["rho"])}
+    comps = [0, 1, 2] if cs.mesh.dim == 3 else [0, 1]
+    for i in comps:
+        out[f"u{i}"] = rel_l2(md["u"][:, i], mo["u"][:, i])
+    if cs.model == capi.MODEL_CDUGKS_SHAKHOV:
+        out["T"] = rel_l2(md["T"], mo["T"])
+        for i in comps:
+            out[f"q{i}"] = rel_l2(md["q"][:, i], mo["q"][:, i])
+    return out
+
+
+# velocity sets of 64 or more (the patch kernels), walls, farfield and
+# pressure boundaries, 2D and 3D, with and without the limiter
+FP32X_CASES = {"shak_quad6": 200, "shak_hex3": 200, "shak_quad12_nc24": 200, "shak_kn1_quad10": 200,
+               "shak_quad6_lim": 200, "shak_channel_pressure": 200, "shak_channel_farfield": 200,
+               "kn0075_cavity": 100}
+
+
+@pytest.mark.parametrize("name", list(FP32X_CASES))
+def test_fp32x_storage_close_to_oracle(case_factory, name):
+    cs = capi.Case(case_factory(name))
+    steps = FP32X_CASES[name]
+    _say(f"[gpu] fp32x {name} NC={cs.n_cells} NV={cs.n_vel} steps={steps}")
+    dev = capi.Device(cs, capi.STORAGE_FP32X)
+    orc = Oracle(cs)
+    dev.init_equilibrium()
+    orc.init_equilibrium()
+    dev.step(steps)
+    orc.step(steps)
+    err = macro_errors(cs, dev.macros(), orc.macros())
+    _say(f"[gpu] fp32x {name}: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()))
+    for k, v in err.items():
+        assert v < FP32X_TOL, f"{name} {k}: {v:.2e}"
+
+
+def test_fp32x_state_roundtrip(case_factory):
+    """set_state / get_state keep 31 mantissa bits of the deviation from f_ref."""
+    cs = capi.Case(case_factory("shak_kn1_quad10"))
+    orc = Oracle(cs)
+    orc.init_equilibrium()
+    orc.step(5)
+    f = orc.get_state()
+    m = orc.macros()
+    dev = capi.Device(cs, capi.STORAGE_FP32X)
+    dev.set_state(f, m["rho"], m["u"], m["T"], m["q"])
+    back = dev.get_state()
+    # the deviation from the rest Maxwellian is at most ~f, so 2^-31 of f bounds the error
+    assert np.abs(back - f).max() <= 2.0 ** -30 * np.abs(f).max()
+    dev32 = capi.Device(cs, capi.STORAGE_FP32)
+    dev32.set_state(f, m["rho"], m["u"], m["T"], m["q"])
+    e32 = np.abs(dev32.get_state() - f).max()
+    assert np.abs(back - f).max() < e32 / 64  # 8 more bits than FP32
+    dev.step(5)
+    orc.step(5)
+    err = macro_errors(cs, dev.macros(), orc.macros())
+    for k, v in err.items():
+        assert v < FP32X_TOL, f"{k}: {v:.2e}"
+
+
+def test_fp32x_graph_replays_match_eager(case_factory):
+    cs = capi.Case(case_factory("shak_quad12_nc24"))
+    out = []
+    for graphs in (False, True):
+        dev = capi.Device(cs, capi.STORAGE_FP32X)
+        dev.set_graphs(graphs)
+        dev.init_equilibrium()
+        dev.step(6)
+        dev.step(1)
+        out.append((dev.get_state(), dev.macros()))
+        dev.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    for k in ("rho", "u", "T", "q"):
+        assert np.array_equal(out[0][1][k], out[1][1][k])
+
+
+def test_fp32x_rejects_per_cell_kernel_cases(case_factory):
+    """FP32X runs on the patch kernels only: a 36-velocity set is refused."""
+    cs = capi.Case(case_factory("shak_quad6_nc6_lim"))
+    with pytest.raises(capi.MesoError):
+        capi.Device(cs, capi.STORAGE_FP32X)
+
+
+def test_fp32x_closer_than_fp32(case_factory):
+    """Same case, same steps: FP32X sits well below FP32 storage."""
+    cs = capi.Case(case_factory("shak_kn1_quad10"))
+    orc = Oracle(cs)
+    orc.init_equilibrium()
+    orc.step(300)
+    mo = orc.macros()
+    errs = {}
+    for st in (capi.STORAGE_FP32, capi.STORAGE_FP32X):
+        dev = capi.Device(cs, st)
+        dev.init_equilibrium()
+        dev.step(300)
+        errs[st] = max(macro_errors(cs, dev.macros(), mo).values())
+    _say(f"[gpu] 300 steps: fp32 {errs[capi.STORAGE_FP32]:.2e} fp32x {errs[capi.STORAGE_FP32X]:.2e}")
+    assert errs[capi.STORAGE_FP32X] < errs[capi.STORAGE_FP32] / 10
