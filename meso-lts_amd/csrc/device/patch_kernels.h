@@ -114,25 +114,14 @@ __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
 
 // LSQ gradient g[0..D-1] of one item from the staged rows of distribution d:
 // sum_j a_j (f_j - f_c) (mesh_numerical.cpp:24-49).
-// A staged value in FP64. mid is 0, or (FP32X storage, whose staged words
-// lack their extension bytes) the midpoint bit of the missing byte, so that
-// the word reads without bias (x32_mid).
-template <class S>
-__device__ __forceinline__ double staged(S x, unsigned mid) {
-    if constexpr (sizeof(S) == 4)
-        return __longlong_as_double((long long)((unsigned long long)__double_as_longlong(double(x)) | mid));
-    else
-        return double(x);
-}
-
 template <class S, int VT, int D>
 __device__ __forceinline__ void item_grad(const meso_patch::ItemT<D> *it, const S *val_d, int vl, double fc,
-                                          double *g, unsigned mid) {
+                                          double *g) {
 #pragma unroll
     for (int k = 0; k < D; ++k) g[k] = 0.0;
 #pragma unroll
     for (int j = 0; j < meso_patch::Lim<D>::MAXN; ++j) {
-        const double df = staged(val_d[it->nb[j] * VT + vl], mid) - fc;
+        const double df = double(val_d[it->nb[j] * VT + vl]) - fc;
 #pragma unroll
         for (int k = 0; k < D; ++k) g[k] += it->a[j][k] * df;
     }
@@ -142,11 +131,11 @@ __device__ __forceinline__ void item_grad(const meso_patch::ItemT<D> *it, const 
 // staged rows (padding slots repeat the item); for the limiter.
 template <class S, int VT, int D>
 __device__ __forceinline__ void item_range(const meso_patch::ItemT<D> *it, const S *val_d, int vl, double fc,
-                                           double &mn, double &mx, unsigned mid) {
+                                           double &mn, double &mx) {
     mn = mx = fc;
 #pragma unroll
     for (int j = 0; j < meso_patch::Lim<D>::MAXN; ++j) {
-        const double w = staged(val_d[it->nb[j] * VT + vl], mid);
+        const double w = double(val_d[it->nb[j] * VT + vl]);
         mn = fmin(mn, w);
         mx = fmax(mx, w);
     }
@@ -158,6 +147,39 @@ __device__ __forceinline__ void stage_copy(const R *src, int n, R *dst) {
     const int4 *si = reinterpret_cast<const int4 *>(src);
     int4 *di = reinterpret_cast<int4 *>(dst);
     for (int r = threadIdx.x; r < n * int(sizeof(R) / 16); r += BLOCKP) di[r] = si[r];
+}
+
+// FP32X storage (XS kernels, FP64 arithmetic): the rows hold the FP32 word and
+// the extension byte of the deviation from f_ref; they are staged as FP64
+// full values (decoded, f_ref of the tile velocity added back).
+template <int ND, int VT>
+__device__ __forceinline__ void stage_values_x(const KArgs &a, const int *rows, int nr, int nvp, int v, bool ok,
+                                               const double ref[2], int slot, int vl, double *val, int maxr) {
+    constexpr int NSLOT = (BLOCKP / 64) * (64 / VT);
+    constexpr int B = 8;
+    const float *wg = (const float *)a.fin_g, *wh = (const float *)a.fin_h;
+    for (int base = slot; base < nr; base += NSLOT * B) {
+        float w[B][ND];
+        unsigned char e[B][ND];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int r = base + k * NSLOT;
+            const bool on = ok && r < nr;
+            const size_t o = size_t(on ? rows[r] : 0) * nvp + (on ? v : 0);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                w[k][d] = on ? (d == 0 ? wg : wh)[o] : 0.0f;
+                e[k][d] = on ? (d == 0 ? a.xin_g : a.xin_h)[o] : 0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int r = base + k * NSLOT;
+            if (r < nr)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) val[(d * maxr + r) * VT + vl] = ok ? x32_decode(w[k][d], e[k][d]) + ref[d] : 0.0;
+        }
+    }
 }
 
 // Patch records: with one item per wave (IPW == 1) every record a wave reads
@@ -389,7 +411,9 @@ __device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn,
 //     that side (tile = side * p_chunks + chunk; the slot of a side the patch
 //     does not own is written by the patch that owns it, a face with both
 //     sides owned puts its sum in slot 0 and zeros in slot 1).
-template <int MODEL, class S, int VT, int D, bool LIM, bool CELLS>
+// XS: FP32X storage (S = double): rows and f-bar+ in HBM as FP32 word and
+// extension byte of the deviation from f_ref.
+template <int MODEL, class S, int VT, int D, bool LIM, bool CELLS, bool XS = false>
 __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.bad_count > 0) {
@@ -446,7 +470,6 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     const int nvp = a.s.nvp;
     const double hdt = a.s.half_dt;
     const S *fin_g = (const S *)a.fin_g, *fin_h = (const S *)a.fin_h;
-    const unsigned mid = a.xin_g ? (1u << 28) : 0u;  // FP32X: read the words without bias
     // phase-C layout: thread = (face, velocity subset); FG faces per subset
     // (FG = the patch's faces, nvs = BLOCKP / FG subsets; the last
     // BLOCKP mod FG threads idle)
@@ -464,7 +487,14 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         const bool ok = v < a.s.nv;
         const double4 X = a.vel_d[ok ? v : 0];
         // A: stage values and tile velocities, clear the face rows
-        stage_values<ND, S, VT>(rows, nr, fin_g, fin_h, nvp, v, ok, slot, vl, val, maxr);
+        double ref[2] = {0.0, 0.0};  // XS: f_ref of the tile velocity (g, h)
+        if constexpr (XS) {
+            ref[0] = ok ? a.fref_d[v] : 0.0;
+            ref[1] = ok && SH ? a.fref_d[a.s.nv + v] : 0.0;
+            stage_values_x<ND, VT>(a, rows, nr, nvp, v, ok, ref, slot, vl, (double *)val, maxr);
+        } else {
+            stage_values<ND, S, VT>(rows, nr, fin_g, fin_h, nvp, v, ok, slot, vl, val, maxr);
+        }
         if (threadIdx.x < VT) {
             double *y = vtd + threadIdx.x * VW;
             if constexpr (D == 2) {
@@ -494,11 +524,11 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
             double fc[2], g[2][D], mn[2], mx[2];
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
-                fc[d] = staged(val[(d * maxr + s) * VT + vl], mid);
+                fc[d] = double(val[(d * maxr + s) * VT + vl]);
 #pragma unroll
                 for (int k = 0; k < D; ++k) g[d][k] = 0.0;
-                if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d], mid);
-                if constexpr (LIM) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d], mid);
+                if (a.s.grad) item_grad<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], g[d]);
+                if constexpr (LIM) item_range<S, VT, D>(it, val + d * maxr * VT, vl, fc[d], mn[d], mx[d]);
             }
             const double omega = LIM ? venkata_omega(a.s.venkata_k, a.cell_vol[it->cell], D) : 0.0;
             const double xv[3] = {X.x, X.y, X.z};
@@ -576,12 +606,20 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                     // (an item's lanes are consecutive velocities of one
                     // face); interior dead-band values are not needed there
                     // (no flux)
-                    S *fo = (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h) + size_t(fg) * nvp + v;
+                    const size_t fi = size_t(fg) * nvp + v;
                     if (mode == meso_patch::SIDE_BOUNDARY || up) {
                         const S fv = S(mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
                                                                                  : fc[d] + Xd);
                         *dst = fv;
-                        *fo = fv;
+                        if constexpr (XS) {
+                            float w;
+                            unsigned char e;
+                            x32_encode(double(fv) - ref[d], w, e);
+                            ((float *)(d == 0 ? a.fbar_g : a.fbar_h))[fi] = w;
+                            (d == 0 ? a.xbar_g : a.xbar_h)[fi] = e;
+                        } else {
+                            (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h)[fi] = fv;
+                        }
                     } else if (dead) {
                         atomicAdd(dst, S(0.5 * (fc[d] + Xd)));
                     }
@@ -676,7 +714,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
 // copied to LDS once per block (read per lane from HBM in every tile they
 // would cost address arithmetic and L2 latency). LIM only selects the
 // instance that pairs with the limited moment kernel.
-template <int MODEL, class S, int VT, bool TAB, int D, bool LIM>
+template <int MODEL, class S, int VT, bool TAB, int D, bool LIM, bool XS = false>
 __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.halt) return;
@@ -735,6 +773,8 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
     }
     const int ntiles = (a.s.nv + VT - 1) / VT;
     const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
+    // XS: seed of the stochastic rounding of the stored values (the device step counter)
+    const unsigned seed = XS ? mix32(unsigned(*a.step_ctr) * 0x9e3779b9u + 0x632be5abu) : 0u;
     // Shakhov heat-flux partials of this thread's owned cells over the chunk
     constexpr int UPD_QJ = upd_qj(VT);
     double qreg[UPD_QJ][D];
@@ -748,7 +788,14 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
         const LaneVel L = lane_vel<D>(a, v);
         const double xv[3] = {L.x, L.y, L.z};
         // A: stage the owned cells' values (read in C only: no barrier before B)
-        stage_values<ND, S, VT>(rows, nown, fin_g, fin_h, nvp, v, L.ok, slot, vl, val, maxr);
+        double ref[2] = {0.0, 0.0};  // XS: f_ref of the lane velocity (g, h)
+        if constexpr (XS) {
+            ref[0] = L.ok ? a.fref_d[v] : 0.0;
+            ref[1] = L.ok && SH ? a.fref_d[a.s.nv + v] : 0.0;
+            stage_values_x<ND, VT>(a, rows, nown, nvp, v, L.ok, ref, slot, vl, (double *)val, maxr);
+        } else {
+            stage_values<ND, S, VT>(rows, nown, fin_g, fin_h, nvp, v, L.ok, slot, vl, val, maxr);
+        }
         // B: per face and velocity (all lanes busy): f-bar+ (written by the
         // moment kernel), its equilibrium mix,
         // the boundary value, flux (xi.n) A f_face into the same slot
@@ -796,8 +843,13 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                     const int l = lf + k * FS < nfp ? lf + k * FS : lf;
                     const int fk = faces[l].face;
                     const size_t o = size_t(fk) * nvp + v;
-                    fbv[k][0] = fbg[o];
-                    fbv[k][1] = SH ? fbh[o] : S(0);
+                    if constexpr (XS) {
+                        fbv[k][0] = S(x32_decode(((const float *)fbg)[o], a.xbar_g[o]) + ref[0]);
+                        fbv[k][1] = SH ? S(x32_decode(((const float *)fbh)[o], a.xbar_h[o]) + ref[1]) : S(0);
+                    } else {
+                        fbv[k][0] = fbg[o];
+                        fbv[k][1] = SH ? fbh[o] : S(0);
+                    }
                     if constexpr (PF) {
                         const S *tb = ftab + size_t(fk) * a.nt;
                         tav[k][0] = tb[L.vt & 1023];
@@ -824,14 +876,6 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             // Shakhov: the cell's axis-table entries (tables of g_S^n and
             // g_S^{n+1}, in the storage type) are read first, so that their
             // latency overlaps the flux sums
-            unsigned char xe[2] = {0, 0};  // FP32X: extension bytes of f^n
-            if constexpr (sizeof(S) == 4) {
-                if (a.xin_g && L.ok) {
-                    const size_t o = size_t(c) * nvp + v;
-                    xe[0] = a.xin_g[o];
-                    if (SH) xe[1] = a.xin_h[o];
-                }
-            }
             S ta[2][3];
             if constexpr (TAB && SH) {
                 const S *tb = reinterpret_cast<const S *>(a.cell_tab) + size_t(c) * 2 * a.nt;
@@ -857,14 +901,8 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             }
             const CellRec<S, D> &R = crl[s];
             double qv[3] = {0.0, 0.0, 0.0};
-            double fcg = double(val[s * VT + vl]);
-            double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
-            if constexpr (sizeof(S) == 4) {
-                if (a.xin_g) {
-                    fcg = x32_decode(val[s * VT + vl], xe[0]);
-                    if (SH) fch = x32_decode(val[(maxr + s) * VT + vl], xe[1]);
-                }
-            }
+            const double fcg = double(val[s * VT + vl]);
+            const double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
             double og, oh = 0.0;
             if (SH) {
                 double gsn, hsn, gs, hs;
@@ -945,23 +983,21 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             }
             if (L.ok) {
                 const size_t o = size_t(c) * nvp + v;
-                bool done = false;
-                if constexpr (sizeof(S) == 4) {
-                    if (a.xout_g) {
-                        float wd;
-                        unsigned char e;
-                        x32_encode(og, wd, e);
-                        fout_g[o] = wd;
-                        a.xout_g[o] = e;
-                        if (SH) {
-                            x32_encode(oh, wd, e);
-                            fout_h[o] = wd;
-                            a.xout_h[o] = e;
-                        }
-                        done = true;
+                if constexpr (XS) {
+                    // deviation from f_ref, stochastically rounded at bit 31
+                    // (random bits from the entry and the step)
+                    float wd;
+                    unsigned char e;
+                    const unsigned rg = mix32(unsigned(o) ^ seed);
+                    x32_encode_sr(og - ref[0], rg, wd, e);
+                    ((float *)fout_g)[o] = wd;
+                    a.xout_g[o] = e;
+                    if (SH) {
+                        x32_encode_sr(oh - ref[1], mix32(rg), wd, e);
+                        ((float *)fout_h)[o] = wd;
+                        a.xout_h[o] = e;
                     }
-                }
-                if (!done) {
+                } else {
                     fout_g[o] = S(og);
                     if (SH) fout_h[o] = S(oh);
                 }

@@ -162,22 +162,24 @@ __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restri
                                                double sn, int v, const double xh[3], double eq_full, double mix) {
     const int nvp = a.s.nvp;
     if constexpr (sizeof(T) == 8) {
-        // FP64 storage with the patch kernels: the moment kernel wrote this
-        // face's f-bar+ (the same reconstruction, unrounded); FP32 storage
-        // rounds it, so it reconstructs here in FP64
+        // FP64 and FP32X storage with the patch kernels: the moment kernel
+        // wrote this face's f-bar+ (the same reconstruction; FP32X keeps 31
+        // mantissa bits of its deviation from f_ref); FP32 storage rounds it,
+        // so it reconstructs here in FP64
         if (a.fbar_g) {
-            const T *fbr = k == 0 ? (const T *)a.fbar_g : (const T *)a.fbar_h;
-            return (1.0 - mix) * double(fbr[size_t(f) * nvp + v]) + mix * eq_full;
+            const size_t i = size_t(f) * nvp + v;
+            double fbar;
+            if (a.xbar_g) {
+                const float *w = (const float *)(k == 0 ? a.fbar_g : a.fbar_h);
+                fbar = x32_decode(w[i], (k == 0 ? a.xbar_g : a.xbar_h)[i]) + a.fref_d[k * a.s.nv + v];
+            } else {
+                fbar = double((k == 0 ? (const T *)a.fbar_g : (const T *)a.fbar_h)[i]);
+            }
+            return (1.0 - mix) * fbar + mix * eq_full;
         }
     }
-    // FP32X storage: every stored word with its extension byte
-    const unsigned char *xe = sizeof(T) == 4 ? (k == 0 ? a.xin_g : a.xin_h) : nullptr;
-    auto ld = [&](size_t i) -> double {
-        if constexpr (sizeof(T) == 4) {
-            if (xe) return x32_decode(float(fd[i]), xe[i]);
-        }
-        return double(fd[i]);
-    };
+    // (FP32X storage always runs the patch kernels, so its values come from f-bar+ above)
+    auto ld = [&](size_t i) -> double { return double(fd[i]); };
     const double fs = ld(size_t(c) * nvp + v);
     double fb;
     const bool dead = !(sn >= VSMALL || sn < -VSMALL);
@@ -1382,19 +1384,18 @@ __global__ __launch_bounds__(BLOCK) void k_dugks_tobp(KArgs a) {
 }
 
 // initial(): f from the patch values, macros from the discrete moments
-// Stores v at entry i of a distribution array: the FP32X word and its
-// extension byte when x is given, else the storage type.
+// Stores v at entry i of a distribution array: with an extension array x
+// (FP32X storage) the array holds FP32 words and v goes in as word and
+// byte, else as the storage type.
 template <class T>
 __device__ __forceinline__ void st_val(T *p, unsigned char *x, size_t i, double v) {
-    if constexpr (sizeof(T) == 4) {
-        if (x) {
-            float w;
-            unsigned char e;
-            x32_encode(v, w, e);
-            p[i] = w;
-            x[i] = e;
-            return;
-        }
+    if (x) {
+        float w;
+        unsigned char e;
+        x32_encode(v, w, e);
+        reinterpret_cast<float *>(p)[i] = w;
+        x[i] = e;
+        return;
     }
     p[i] = T(v);
 }
@@ -1419,11 +1420,11 @@ __global__ __launch_bounds__(BLOCK) void k_init_eq(KArgs a, const double *rp, co
             const size_t o = size_t(c) * a.s.nvp + v;
             if (MODEL == CDUGKS_SHAKHOV) {
                 shakhov_eq<double>(e, X, D, a.s.K, g, h);
-                st_val((T *)a.fout_h, a.xout_h, o, h - (sizeof(T) == 4 ? a.fref_d[a.s.nv + v] : 0.0));
+                st_val((T *)a.fout_h, a.xout_h, o, h - (sizeof(T) == 4 || a.xout_h ? a.fref_d[a.s.nv + v] : 0.0));
             } else {
                 g = bgk_eq<double>(e, X);
             }
-            st_val((T *)a.fout_g, a.xout_g, o, g - (sizeof(T) == 4 ? a.fref_d[v] : 0.0));
+            st_val((T *)a.fout_g, a.xout_g, o, g - (sizeof(T) == 4 || a.xout_g ? a.fref_d[v] : 0.0));
             const double kk = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
             const double c0 = X[0] - u[0], c1 = X[1] - u[1], c2 = X[2] - u[2];
             const double cc = c0 * c0 + c1 * c1 + c2 * c2;
@@ -1470,14 +1471,15 @@ __global__ __launch_bounds__(BLOCK) void k_init_macros(KArgs a) {
     }
     for (int v = li; v < a.s.nv; v += LP) {
         const double X[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
-        const double rg = sizeof(T) == 4 ? a.fref_d[v] : 0.0;
+        const bool dev = sizeof(T) == 4 || a.xout_g;  // deviation from f_ref stored
+        const double rg = dev ? a.fref_d[v] : 0.0;
         const size_t o = size_t(c) * a.s.nvp + v;
         T *g0 = (T *)a.fout_g, *h0 = (T *)a.fout_h;
         if (MODEL == CDUGKS_SHAKHOV) {
             double g, h;
             shakhov_eq<double>(e, X, D, a.s.K, g, h);
             st_val(g0, a.xout_g, o, g - rg);
-            st_val(h0, a.xout_h, o, h - (sizeof(T) == 4 ? a.fref_d[a.s.nv + v] : 0.0));
+            st_val(h0, a.xout_h, o, h - (dev ? a.fref_d[a.s.nv + v] : 0.0));
         } else {
             st_val(g0, a.xout_g, o, bgk_eq<double>(e, X) - rg);
         }
@@ -1626,7 +1628,7 @@ __global__ __launch_bounds__(BLOCK) void k_mask_repair(KArgs a) {
                 double g, h;
                 shakhov_eq<double>(e, X, D, a.s.K, g, h);
                 const size_t o = size_t(c) * a.s.nvp + v;
-                const bool dev = sizeof(T) == 4;
+                const bool dev = sizeof(T) == 4 || a.xout_g;
                 st_val((T *)a.fout_g, a.xout_g, o, g - (dev ? a.fref_d[v] : 0.0));
                 st_val((T *)a.fout_h, a.xout_h, o, h - (dev ? a.fref_d[a.s.nv + v] : 0.0));
             }
@@ -1746,10 +1748,12 @@ struct meso_hip_ctx {
     meso_mark_desc *marks = nullptr;
     double *xi = nullptr, *w = nullptr;
     void *f[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [buffer][distribution]
-    // FP32X storage (storage is then MESO_STORAGE_FP32 with ext set): the
-    // extension bytes of f, [buffer][distribution]
+    // FP32X storage (storage is then MESO_STORAGE_FP64, the kernels' arithmetic,
+    // with ext set): f holds FP32 words of the deviation from f_ref and fx the
+    // extension bytes, [buffer][distribution]
     bool ext = false;
     unsigned char *fx[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    unsigned char *xbar[2] = {nullptr, nullptr};  // FP32X: extension bytes of f-bar+
     // masked cells (meso_hip_set_mask): 1 per cell of a group with `mask switch True`
     int *mask = nullptr;
     bool has_mask = false;
@@ -1918,6 +1922,8 @@ KArgs make_args(meso_hip_ctx *c) {
     a.fin_h = c->f[i][1];
     a.fout_g = c->f[o][0];
     a.fout_h = c->f[o][1];
+    a.xbar_g = c->xbar[0];
+    a.xbar_h = c->xbar[1];
     a.xin_g = c->fx[i][0];
     a.xin_h = c->fx[i][1];
     a.xout_g = c->fx[o][0];
@@ -2044,7 +2050,7 @@ int set_reference(meso_hip_ctx *c, double rho_ref, double T_ref) {
 
 int set_reference_from(meso_hip_ctx *c, const double *rho, const double *T) {
     ++c->kver;  // the reference state is a kernel argument
-    if (c->storage != MESO_STORAGE_FP32) return MESO_OK;
+    if (c->storage != MESO_STORAGE_FP32 && !c->ext) return MESO_OK;
     const int nc = c->s.nc;
     double r = 0.0, t = 0.0;
     for (int i = 0; i < nc; ++i) {
@@ -2096,7 +2102,7 @@ struct Launch {
 
 // Step with the LDS-staged patch kernels for the face moments and the
 // interior cells; boundary faces and mirror-boundary cells keep their kernels.
-template <int MODEL, class T, bool TAB, int D, bool LIM>
+template <int MODEL, class T, bool TAB, int D, bool LIM, bool X>
 int step_patch(meso_hip_ctx *c, KArgs a) {
     const StepConst &s = c->s;
     constexpr int WPB = BLOCK / 64;
@@ -2106,19 +2112,19 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
     const int fwblk = (s.nf + WPB - 1) / WPB;
     if (c->mp_cells) {
         if (!c->mp_attr[1][LIM] && c->lds_mp > 65536) {
-            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, true>),
+            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, true, X>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_mp)));
             c->mp_attr[1][LIM] = true;
         }
-        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, true>), c->n_mp * c->p_chunks, c->lds_mp,
+        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, true, X>), c->n_mp * c->p_chunks, c->lds_mp,
                  a);
     } else {
         if (!c->mp_attr[0][LIM] && c->lds_mp > 65536) {
-            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, false>),
+            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, false, X>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_mp)));
             c->mp_attr[0][LIM] = true;
         }
-        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, false>), c->n_mp * c->p_chunks,
+        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, false, X>), c->n_mp * c->p_chunks,
                  c->lds_mp, a);
     }
     LAUNCH_SH(c, "face_eq", (k_face_eq<MODEL, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
@@ -2126,11 +2132,11 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
     if (c->n_int > 0) {
         LAUNCH(c, "cell_prep", (k_cell_prep<MODEL, T, TAB, D>), (c->n_int + WPB - 1) / WPB, a);
         if (!c->upd_attr[LIM] && c->lds_up > 65536) {
-            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_update<MODEL, T, VTU, TAB, D, LIM>),
+            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_update<MODEL, T, VTU, TAB, D, LIM, X>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_up)));
             c->upd_attr[LIM] = true;
         }
-        LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, D, LIM>), c->n_up * c->p_chunks, c->lds_up,
+        LAUNCH_P(c, "cell_update", (k_patch_update<MODEL, T, VTU, TAB, D, LIM, X>), c->n_up * c->p_chunks, c->lds_up,
                  a);
     }
     const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
@@ -2150,7 +2156,12 @@ int step_body(meso_hip_ctx *c, KArgs a) {
     constexpr int MF = MODEL == CDUGKS_SHAKHOV ? CDUGKS_SHAKHOV : CDUGKS_BGK;
     constexpr int MAXF = D == 2 ? 4 : 6;
     if constexpr (MODEL != DUGKS_BGK) {
-        if (c->patches) return step_patch<MODEL, T, TAB, D, LIM>(c, a);
+        if (c->patches) {
+            // FP32X: the FP64 kernels on FP32 words + extension bytes
+            if constexpr (sizeof(T) == 8)
+                if (c->ext) return step_patch<MODEL, T, TAB, D, LIM, true>(c, a);
+            return step_patch<MODEL, T, TAB, D, LIM, false>(c, a);
+        }
     }
     const int fwblk = (s.nf + WPB - 1) / WPB;
     LAUNCH_SH(c, "face_moments", (k_face_moments<MF, D, T, LIM>), fwblk * s.ntile_f, 0, a);
@@ -2479,9 +2490,9 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     if (storage != MESO_STORAGE_FP64 && storage != MESO_STORAGE_FP32 && storage != MESO_STORAGE_FP32X)
         return set_err(nullptr, MESO_E_INVALID,
                        "storage must be MESO_STORAGE_FP64, MESO_STORAGE_FP32 or MESO_STORAGE_FP32X");
-    // FP32X runs the FP32 kernels with the extension bytes beside the FP32 words
+    // FP32X runs the FP64 kernels on words + extension bytes in HBM
     const bool ext = storage == MESO_STORAGE_FP32X;
-    if (ext) storage = MESO_STORAGE_FP32;
+    if (ext) storage = MESO_STORAGE_FP64;
     if (m->n_cells <= 0 || m->n_faces <= 0 || d->n_vel <= 0) return set_err(nullptr, MESO_E_INVALID, "empty mesh or DVS");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -2914,11 +2925,12 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     if (!identity) UP(slot_of, slot_of.data(), slot_of.size());
 #undef UP
     const size_t es = storage == MESO_STORAGE_FP32 ? 4 : 8;
+    const size_t fes = ext ? 4 : es;  // bytes per stored distribution value (FP32X: the word; bytes apart)
     const size_t fsz = size_t(nc) * c->s.nvp;
     for (int b = 0; b < 2; ++b)
         for (int k = 0; k < c->nd; ++k) {
             char *q = nullptr;
-            if ((rc = dalloc(c, &q, fsz * es))) return fail(rc);
+            if ((rc = dalloc(c, &q, fsz * fes))) return fail(rc);
             c->f[b][k] = q;
             if (ext && (rc = dalloc(c, &c->fx[b][k], fsz))) return fail(rc);
         }
@@ -2958,8 +2970,9 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         }
         for (int k = 0; k < c->nd; ++k) {
             char *fb = nullptr;
-            if ((rc = dalloc(c, &fb, size_t(nf) * c->s.nvp * es))) return fail(rc);
+            if ((rc = dalloc(c, &fb, size_t(nf) * c->s.nvp * fes))) return fail(rc);
             c->fbar[k] = fb;
+            if (ext && (rc = dalloc(c, &c->xbar[k], size_t(nf) * c->s.nvp))) return fail(rc);
         }
     }
     c->xi_h = xi_s;
@@ -3096,7 +3109,8 @@ static int transpose(meso_hip_ctx *c, bool to_device, double *host, int which, i
     dim3 grid((nc + 31) / 32, (nv + 31) / 32);
     void *dev = c->f[c->cur][which];
     unsigned char *xdev = c->fx[c->cur][which];
-    const bool f32 = c->storage == MESO_STORAGE_FP32;
+    // FP32X: FP32 words of the deviation plus extension bytes (float instantiation)
+    const bool f32 = c->storage == MESO_STORAGE_FP32 || c->ext;
     const double *fref = f32 ? c->fref_d + size_t(which) * nv : nullptr;
     if (to_device) {
         HIP_TRY(c, hipMemcpyAsync(stage, src, n * 8, hipMemcpyHostToDevice, c->stream));

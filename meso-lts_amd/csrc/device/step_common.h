@@ -41,13 +41,30 @@ __device__ __forceinline__ void x32_encode(double v, float &hi, unsigned char &e
     ext = (unsigned char)((b >> 21) & 0xffu);
     hi = float(__longlong_as_double((long long)(b & ~((1ull << 29) - 1))));  // exact
 }
+// The same with stochastic rounding at bit 31: r holds 21 random bits, so the
+// value rounds up with the probability of its position between the two
+// neighbours. Unbiased, so that the small per-step increments of a run near
+// its steady state are not lost the way round-to-nearest loses the ones below
+// half a unit in the last place.
+__device__ __forceinline__ void x32_encode_sr(double v, unsigned r, float &hi, unsigned char &ext) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v) + (r & ((1u << 21) - 1u));
+    ext = (unsigned char)((b >> 21) & 0xffu);
+    hi = float(__longlong_as_double((long long)(b & ~((1ull << 29) - 1))));
+}
+// 32-bit mixing function (xorshift-multiply; "lowbias32" constants): the
+// random bits of the stochastic rounding, from (entry index, step seed).
+__host__ __device__ __forceinline__ unsigned mix32(unsigned x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
 __device__ __forceinline__ double x32_decode(float hi, unsigned ext) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(double(hi));
     return __longlong_as_double((long long)(b | ((unsigned long long)(ext & 0xffu) << 21)));
 }
-// The word alone, without bias: the midpoint of the values its byte can add
-// (the reconstruction reads FP32 words; the update reads word and byte).
-__device__ __forceinline__ double x32_mid(float hi) { return x32_decode(hi, 0x80u); }
 
 // Per-face equilibrium description, written by the face-macro kernel.
 // Shakhov: g_S = A exp(-B|c|^2) [1 + P1 (c.q)(|c|^2 iRT - D - 2)]

@@ -96,8 +96,9 @@ struct KArgs {
     // state
     const void *fin_g, *fin_h;  // cell-major, f^n
     void *fout_g, *fout_h;      // cell-major, f^{n+1}
-    // FP32X storage: the extension byte of every FP32 word of f^n / f^{n+1}
-    // (same indexing); null for FP32 and FP64 storage
+    // FP32X storage: f^n / f^{n+1} hold FP32 words of the deviation from f_ref
+    // and these arrays the extension byte of every word (same indexing); null
+    // for FP32 and FP64 storage
     const unsigned char *xin_g, *xin_h;
     unsigned char *xout_g, *xout_h;
     const double *rho_in, *u_in, *T_in, *q_in;
@@ -135,6 +136,7 @@ struct KArgs {
     // patch kernels: f-bar+ of every face, written by the moment kernel and
     // read by the update kernel (face-major [nf][nvp], storage type)
     void *fbar_g, *fbar_h;
+    unsigned char *xbar_g, *xbar_h;  // FP32X: extension bytes of f-bar+ (null otherwise)
     void *cell_tab;             // [nc][2][nt] axis tables of the cell equilibria
     // FP32 storage keeps f - f_ref; f_ref is the equilibrium `ref` at rest.
     EqParams ref;

@@ -1,0 +1,103 @@
+"""The batched cavity sweep at its stated size: 64 seeded instances (quads 32^2
+to 128^2, Kn log-uniform in [0.01, 10], lid speed U[0.05, 0.3], Newton-Cotes
+16^2 to 64^2 velocities; configs.batch_instances(64)) advanced together with
+meso_hip_step_batch, in FP64, FP32X and FP32 storage.
+
+Checked per instance after 50 steps: mass conservation in the closed cavity,
+FP32X against FP64 within 1e-6 relative L2 on every macro component, FP32
+against FP64 within the FP32 bound of the small instances (2e-6: the coarse
+16^2 velocity sets at low Kn carry a heat flux of ~1e-3 whose rounding floor
+is ~1e-6 after 50 steps), and, on the smallest instances, FP64 against the
+oracle to 1e-10.
+"""
+import numpy as np
+import pytest
+
+from conftest import max_rel, rel_l2
+from meso_hip import capi, configs
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 50
+FP32X_TOL = 1e-6
+FP32_TOL = 2e-6
+MASS_TOL = 1e-11
+
+_results = {}
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def cases(tmp_path_factory):
+    root = str(tmp_path_factory.mktemp("batch64"))
+    insts = configs.batch_instances(64)
+    return [capi.Case(configs.build_instance(i, root)) for i in insts]
+
+
+def components(cs, m):
+    out = {"rho": m["rho"], "T": m["T"]}
+    for i in range(cs.mesh.dim):
+        out[f"u{i}"] = m["u"][:, i]
+        out[f"q{i}"] = m["q"][:, i]
+    return out
+
+
+def run_batch(cases, storage):
+    devs = []
+    for cs in cases:
+        d = capi.Device(cs, storage)
+        d.init_equilibrium()
+        devs.append(d)
+    vols = [cs.arrays()["cell_vol"] for cs in cases]
+    m0 = [float((d.macros()["rho"] * v).sum()) for d, v in zip(devs, vols)]
+    reps = capi.step_batch(devs, STEPS)
+    assert all(r.steps_done == STEPS and r.bad_cells == 0 for r in reps)
+    out = []
+    for cs, d, v, mm in zip(cases, devs, vols, m0):
+        m = d.macros()
+        out.append((components(cs, m), abs(float((m["rho"] * v).sum()) - mm) / mm))
+        d.close()
+    return out
+
+
+@pytest.mark.parametrize("storage", [capi.STORAGE_FP64, capi.STORAGE_FP32X, capi.STORAGE_FP32])
+def test_batch64_runs_and_conserves_mass(cases, storage):
+    dvc = sum(cs.n_cells * cs.n_vel for cs in cases)
+    _say(f"[gpu] batch of {len(cases)} storage={storage}: {dvc:.3e} DV-cells")
+    res = run_batch(cases, storage)
+    for k, (_, dm) in enumerate(res):
+        assert dm < MASS_TOL, f"instance {k}: mass change {dm:.2e}"
+    _results[storage] = [m for m, _ in res]
+
+
+@pytest.mark.parametrize("storage,tol", [(capi.STORAGE_FP32X, FP32X_TOL), (capi.STORAGE_FP32, FP32_TOL)])
+def test_batch64_reduced_storage_against_fp64(cases, storage, tol):
+    if capi.STORAGE_FP64 not in _results or storage not in _results:
+        pytest.skip("needs the batch runs above")
+    worst = 0.0
+    for k, (a, b) in enumerate(zip(_results[storage], _results[capi.STORAGE_FP64])):
+        for key in b:
+            err = rel_l2(a[key], b[key])
+            worst = max(worst, err)
+            assert err < tol, f"instance {k} (seed {1000 + k}) {key}: {err:.2e}"
+    _say(f"[gpu] batch of 64, storage {storage} against FP64 after {STEPS} steps: worst {worst:.2e}")
+
+
+def test_batch64_smallest_instances_match_oracle(cases):
+    small = sorted(range(len(cases)), key=lambda k: cases[k].n_cells * cases[k].n_vel)[:3]
+    devs = [capi.Device(cases[k], capi.STORAGE_FP64) for k in small]
+    orcs = [Oracle(cases[k]) for k in small]
+    for d, o in zip(devs, orcs):
+        d.init_equilibrium()
+        o.init_equilibrium()
+    capi.step_batch(devs, 5)
+    for k, d, o in zip(small, devs, orcs):
+        o.step(5)
+        md, mo = d.macros(), o.macros()
+        for key in ("rho", "u", "T", "q"):
+            err = max_rel(md[key], mo[key])
+            assert err < 1e-10, f"instance {k} {key}: {err:.2e}"

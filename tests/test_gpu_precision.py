@@ -1,13 +1,16 @@
-"""FP32X storage (an FP32 word plus an 8-bit extension of its significand per
-stored value) on the GPU against the FP64 oracle, and its state transfers.
+"""FP32X storage on the GPU against the FP64 oracle, and its state transfers.
 
-Tolerance: relative L2 <= 1e-7 on every macro component after 200 steps,
-ten times below the 1e-6 the FP32-storage precision target allows. An FP64 CPU
-restatement that rounds the stored values to 31 mantissa bits, the face values
-to FP32 and reads the stored words for the reconstruction (the FP32X step)
-stays below 1e-7 on a 20x20 Kn = 1 cavity with 21^2 velocities through its
-whole run to a steady state, where FP32 storage with round-to-nearest
-stagnates at 1e-4.
+FP32X keeps every distribution value in HBM as an FP32 word plus one byte
+that extends its significand to 31 bits (the deviation from the rest
+Maxwellian f_ref): the cell state between steps (stochastically rounded at bit
+31) and the face values f-bar+ handed from the moment to the update kernel
+(rounded to nearest). All arithmetic is FP64, the step of FP64 storage.
+
+Tolerance: relative L2 <= 1e-7 on every macro component after 100-200 steps,
+ten times below the 1e-6 the precision target allows. (On the Kn = 1 cavity at
+half resolution, run to meso's convergence rule, FP32X stays at ~4e-9 of the
+FP64 run and converges at the same step; FP32 storage with round-to-nearest
+stagnates at 5e-4.)
 """
 import numpy as np
 import pytest
