@@ -2,19 +2,30 @@
 """Benchmark of the gfx950 Shakhov CDUGKS step on the Kn = 1 cavity.
 
 Workload (one GPU): 200x200 quads jittered by 10% (40,000 cells), 101x101
-Newton-Cotes velocities on [-5, 5]^2 (10,201), Shakhov model, FP32 storage of
-g, h with FP64 moments/macros. A step = one full time step (all kernels) of
-the whole mesh; value = DV-cell updates/s = NC * NV * steps / seconds, inputs
-resident in HBM, timed with a device synchronisation on both sides.
+Newton-Cotes velocities on [-5, 5]^2 (10,201), Shakhov model. A step = one
+full time step (all kernels) of the whole mesh; value = DV-cell updates/s =
+NC * NV * steps / seconds, inputs resident in HBM, timed with a device
+synchronisation on both sides.
 
-Also reported: the FP64-storage step of the same case (ratio FP32/FP64), the
-FP32-vs-FP64 relative L2 of every macro after the run, per-kernel times from
-HIP events, the roofline of the dominant kernel (16 B per DV-cell update:
-read + write of g and h in FP32) and the CPU baseline (the FP64 oracle with
-OpenMP on this host, one step of the same workload).
+The step runs in three storages of the distributions g, h:
+  fp32x  an FP32 word plus a byte extending its significand to 31 bits (5 B),
+         FP64 arithmetic: the storage that holds the 1e-6 precision target at
+         convergence; the line's value (--headline picks another)
+  fp32   FP32 words (deviation from the rest Maxwellian), FP64 transport
+         arithmetic, FP32 equilibria
+  fp64   meso's own precision
+Reported per storage: ms/step, DV-cell updates/s, the roofline of the
+dominant kernel (algorithmic bytes: read + write of every stored g, h value of
+the DV-cells the launch owns, over its average launch time from HIP events),
+per-kernel times, and the relative L2 of every macro against FP64 after the
+timed run. at_convergence: the same workload at --converge-scale run to meso's
+stopping rule in all three storages, with the errors there. cpu_baseline: the
+FP64 oracle with OpenMP on this host (one step of the same workload) and its
+measured speed ratio to meso itself.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scale s]
-                       [--storage both|fp32|fp64] [--copy-calibration MIB]
+                       [--storage all|fp32x|fp32|fp64] [--headline fp32x|fp32|fp64]
+                       [--converge-scale s] [--copy-calibration MIB]
                        [--autotune] [--graphs] [--batch N]
        python bench.py --converge [--workload W] [--scale s] [--max-steps N] [--time-limit S] [--run-on]
 
@@ -178,21 +189,27 @@ def macro_errors(a, b, dim, shakhov):
 
 
 def converge_main(args):
+    line = converge_run(args.workload, args.scale, args.max_steps, args.time_limit, args.residual_limit,
+                        args.curve_every, args.run_on)
+    print(json.dumps(line), flush=True)
+
+
+def converge_run(workload, scale, max_steps, time_limit, residual_limit=0.0, curve_every=1000, run_on=False):
     """The workload run to meso's stopping rule in FP64, FP32 and FP32X storage
     side by side: every 100 steps meso's residuals (meso_hip_residual), converged
     after 10 consecutive checks with every residual <= the case's
     residual-limit (cdugks_shakhov.cpp:730-756). Reports the step each storage
     converged at and every macro's FP32 / FP32X error against the FP64 run at
     the FP64 convergence step (and at the end of the run). The run ends when
-    FP64 has converged (with --run-on: when every storage has), or at
-    --max-steps / --time-limit."""
+    FP64 has converged (with run_on: when every storage has), or at
+    max_steps / time_limit."""
     work = os.path.join(tempfile.gettempdir(), "meso_hip_bench")
-    cs = capi.Case(configs.build(args.workload, work, args.scale))
+    cs = capi.Case(configs.build(workload, work, scale))
     dim = cs.mesh.dim
     shakhov = cs.model == capi.MODEL_CDUGKS_SHAKHOV
     limit = cs.setting_double("residual-limit", 1e-6)
-    if args.residual_limit > 0:
-        limit = args.residual_limit
+    if residual_limit > 0:
+        limit = residual_limit
     idx = ([0, 4, 1, 2, 5, 6] if dim == 2 else [0, 4, 1, 2, 3, 5, 6, 7]) if shakhov else list(range(1 + dim))
     kinds = [("fp64", capi.STORAGE_FP64), ("fp32", capi.STORAGE_FP32), ("fp32x", capi.STORAGE_FP32X)]
     devs = {}
@@ -205,7 +222,7 @@ def converge_main(args):
         d.init_equilibrium()
         d.residual()
         devs[tag] = d
-    log(f"[converge] {args.workload} scale {args.scale}: NC={cs.n_cells} NV={cs.n_vel} dt={cs.params.dt:.4e} "
+    log(f"[converge] {workload} scale {scale}: NC={cs.n_cells} NV={cs.n_vel} dt={cs.params.dt:.4e} "
         f"residual-limit {limit:g}")
     state = {t: 0 for t in devs}
     conv_at = {t: None for t in devs}
@@ -214,7 +231,7 @@ def converge_main(args):
     curve = []
     t0 = time.time()
     step = 0
-    while step < args.max_steps and time.time() - t0 < args.time_limit:
+    while step < max_steps and time.time() - t0 < time_limit:
         live = [t for t in devs if conv_at[t] is None]
         if not live:
             break
@@ -231,7 +248,7 @@ def converge_main(args):
             if state[t] >= 10:
                 conv_at[t] = step
         errs = None
-        if step % args.curve_every == 0 or conv_at["fp64"] == step:
+        if step % curve_every == 0 or conv_at["fp64"] == step:
             ref = devs["fp64"].macros()
             errs = {t: macro_errors(devs[t].macros(), ref, dim, shakhov) for t in devs if t != "fp64"}
             curve.append({"step": step, **{t: max(e.values()) for t, e in errs.items()}})
@@ -240,12 +257,12 @@ def converge_main(args):
         log(f"[converge] step {step} " + " ".join(f"{t} res {history[t][-1]:.2e}({state[t]})" for t in live) +
             ("" if errs is None else " | err " + " ".join(f"{t} {max(e.values()):.2e}" for t, e in errs.items())) +
             f" t={time.time() - t0:.0f}s")
-        if conv_at["fp64"] is not None and not args.run_on:
+        if conv_at["fp64"] is not None and not run_on:
             break
     ref = devs["fp64"].macros()
     final = {t: macro_errors(devs[t].macros(), ref, dim, shakhov) for t in devs if t != "fp64"}
     line = {
-        "metric": "FP32-store vs FP64 rel L2 at convergence", "workload": args.workload, "mesh_scale": args.scale,
+        "metric": "FP32-store vs FP64 rel L2 at convergence", "workload": workload, "mesh_scale": scale,
         "cells": cs.n_cells, "velocities": cs.n_vel, "dt": cs.params.dt, "residual_limit": limit,
         "rule": "10 consecutive checks every 100 steps, every residual <= limit (cdugks_shakhov.cpp:730-756)",
         "converged_at": conv_at, "steps_run": step, "seconds": time.time() - t0,
@@ -253,7 +270,9 @@ def converge_main(args):
         "max_error_curve": curve,
         "residual_history_every_100": {t: h[-50:] for t, h in history.items()},
     }
-    print(json.dumps(line), flush=True)
+    for d in devs.values():
+        d.close()
+    return line
 
 
 def main():
@@ -265,8 +284,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--scale", type=float, default=1.0, help="mesh resolution scale (<1: quick runs)")
     ap.add_argument("--workload", default=WORKLOAD)
-    ap.add_argument("--storage", choices=("both", "fp32", "fp64"), default="both",
-                    help="storage kinds to run (profiling runs use one; the JSON line needs both)")
+    ap.add_argument("--storage", choices=("all", "fp32x", "fp32", "fp64"), default="all",
+                    help="storage kinds to run (profiling runs use one; the JSON line needs all)")
+    ap.add_argument("--headline", choices=("fp32x", "fp32", "fp64"), default="fp32x",
+                    help="storage whose rate is the line's value (default: fp32x, the one that holds the "
+                         "1e-6 precision target at convergence)")
+    ap.add_argument("--converge-scale", type=float, default=0.25,
+                    help="also run the workload at this mesh scale to meso's convergence rule in all storages "
+                         "and report the errors there (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
@@ -309,8 +334,8 @@ def main():
 
     res = {}
     macros = {}
-    kinds = [(capi.STORAGE_FP32, "fp32"), (capi.STORAGE_FP64, "fp64")]
-    if args.storage != "both":
+    kinds = [(capi.STORAGE_FP32X, "fp32x"), (capi.STORAGE_FP32, "fp32"), (capi.STORAGE_FP64, "fp64")]
+    if args.storage != "all":
         kinds = [k for k in kinds if k[1] == args.storage]
     tiles = {}
     for storage, tag in kinds:
@@ -330,38 +355,51 @@ def main():
             log(f"          {k:14s} {v['avg_ms']:.4f} ms x{v['launches']}")
         dev.close()
 
-    if args.storage != "both":
+    if args.storage != "all":
         return
-    # precision of the FP32-storage step against the FP64 step (same steps)
-    prec = {}
-    a, b = macros["fp32"], macros["fp64"]
-    prec["rho"] = rel_l2(a["rho"], b["rho"])
     dim = cs.mesh.dim
-    for i, c in enumerate("UVW"[:dim]):
-        prec[c] = rel_l2(a["u"][:, i], b["u"][:, i])
-    if shakhov:
-        prec["T"] = rel_l2(a["T"], b["T"])
-        for i, c in enumerate("xyz"[:dim]):
-            prec["q" + c] = rel_l2(a["q"][:, i], b["q"][:, i])
+    # bytes per stored distribution value: FP32X keeps an FP32 word and a byte
+    sbytes = {"fp32": 4, "fp32x": 5, "fp64": 8}
 
-    s = 4  # FP32 bytes
-    b_alg = 2 * nd * s
-    k32 = res["fp32"]["kernels"]
-    dom = max(k32, key=lambda k: k32[k]["avg_ms"])
-    dom_ms = k32[dom]["avg_ms"]
-    # DV-cells one launch of the dominant kernel processes, as the library
-    # reports them: the update launch owns update_owned cells (every cell
-    # without a mirror boundary); the mirror-boundary cells have their own launch
-    owned = tiles["fp32"]["update_owned"]
-    units = {"cell_update": owned * nv, "cell_update_bnd": (nc - owned) * nv}.get(dom, nc * nv)
-    achieved = b_alg * units / (dom_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as fp:
-            traffic = json.load(fp).get(dom)
-    ms_step = res["fp32"]["seconds"] / args.steps * 1e3
-    step_gbs = b_alg * nc * nv / (ms_step * 1e-3) / 1e9
+    def roofline(tag):
+        """Dominant kernel of a storage: algorithmic bytes (read + write of
+        every distribution value of the DV-cells the launch owns) over its
+        average launch time."""
+        kk = res[tag]["kernels"]
+        dom = max(kk, key=lambda k: kk[k]["avg_ms"])
+        dom_ms = kk[dom]["avg_ms"]
+        # DV-cells one launch of the dominant kernel processes, as the library
+        # reports them: the update launch owns update_owned cells (every cell
+        # without a mirror boundary); the mirror-boundary cells have their own launch
+        owned = tiles[tag]["update_owned"]
+        units = {"cell_update": owned * nv, "cell_update_bnd": (nc - owned) * nv}.get(dom, nc * nv)
+        b_alg = 2 * nd * sbytes[tag]
+        achieved = b_alg * units / (dom_ms * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{tag}.json")
+        if os.path.exists(tpath):
+            with open(tpath) as fp:
+                traffic = json.load(fp).get(dom)
+        return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "alg_bytes_per_dv_cell": b_alg,
+                "alg_bytes_per_launch": b_alg * units, "avg_launch_ms": dom_ms}
+
+    per = {}
+    for tag in res:
+        ms = res[tag]["seconds"] / args.steps * 1e3
+        b_alg = 2 * nd * sbytes[tag]
+        gbs = b_alg * nc * nv / (ms * 1e-3) / 1e9
+        per[tag] = {"ms_per_step": ms, "value": res[tag]["ups"], "roofline": roofline(tag),
+                    "step_hbm": {"alg_bytes_per_step": b_alg * nc * nv, "achieved_GBs": gbs,
+                                 "frac": gbs / HBM_PEAK_GBS},
+                    "kernels_ms": {k: v["avg_ms"] for k, v in res[tag]["kernels"].items()}}
+        if tag != "fp64":
+            per[tag]["vs_fp64_rel_l2_after_run"] = macro_errors(macros[tag], macros["fp64"], dim, shakhov)
+            per[tag]["over_fp64"] = res[tag]["ups"] / res["fp64"]["ups"]
+
+    conv = None
+    if args.converge_scale > 0:
+        conv = converge_run(args.workload, args.converge_scale, 20000, 240.0)
 
     cpu = None
     if not args.no_cpu:
@@ -371,30 +409,31 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
 
+    head = args.headline
+    storage_txt = {"fp32x": "fp32 word + 8-bit significand extension per g,h value (5 B); fp64 arithmetic",
+                   "fp32": "fp32 g,h; fp64 transport arithmetic and moments/macros",
+                   "fp64": "fp64"}
     line = {
         "metric": METRIC,
-        "value": res["fp32"]["ups"],
+        "value": res[head]["ups"],
         "unit": "DV-cell updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": per[head]["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "cells": nc, "faces": cs.n_faces, "velocities": nv,
-                   "distributions": nd, "storage": "fp32 g,h; fp64 moments/macros", "mesh_scale": args.scale,
-                   "graphs": bool(args.graphs), "autotune": bool(args.autotune), "tiles": tiles.get("fp32")},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": b_alg * units, "avg_launch_ms": dom_ms},
-        "step_hbm": {"alg_bytes_per_step": b_alg * nc * nv, "achieved_GBs": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
-        "fp64": {"ms_per_step": res["fp64"]["seconds"] / args.steps * 1e3, "value": res["fp64"]["ups"]},
-        "fp32_over_fp64": res["fp32"]["ups"] / res["fp64"]["ups"],
-        "fp32_vs_fp64_rel_l2": prec,
-        "kernels_ms": {k: v["avg_ms"] for k, v in k32.items()},
+                   "distributions": nd, "storage": storage_txt[head], "headline_storage": head,
+                   "mesh_scale": args.scale, "graphs": bool(args.graphs), "autotune": bool(args.autotune),
+                   "tiles": tiles.get(head)},
+        "roofline": per[head]["roofline"],
+        "step_hbm": per[head]["step_hbm"],
+        "storages": per,
+        "at_convergence": conv,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -38,9 +38,9 @@ def build(name: str, root: str, scale: float = 1.0) -> str:
     """Write workload `name` under root/name and return its case path. `scale`
     (<1) shrinks the mesh resolution for quick runs."""
     w = WORKLOADS[name]
-    d = os.path.join(root, name)
+    d = os.path.join(root, name if scale == 1.0 else f"{name}_s{scale:g}")
     case = os.path.join(d, "case.txt")
-    if os.path.exists(case) and scale == 1.0:
+    if os.path.exists(case):
         return case
     if w["kind"] == "bgk":
         n = max(4, int(w["n"] * scale))
