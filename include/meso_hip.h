@@ -213,6 +213,16 @@ const char *meso_hip_last_error(const meso_hip_ctx *ctx);
  * discrete moments; Shakhov q = m3/2 (cdugks_shakhov.cpp:103-168,
  * cdugks.cpp:67-104, dugks.cpp:68-105). Also resets the residual copies. */
 int meso_hip_init_equilibrium(meso_hip_ctx *ctx, const double *rho, const double *u3, const double *T);
+/* The same initial state without allocating device memory (no hipMalloc,
+ * hipFree or allocating library call): the per-cell macros are staged in a
+ * static 16 MiB device array of the library (one per GPU, held until the
+ * process ends), so the mesh may have at most 419,430 cells; larger meshes are
+ * rejected with MESO_E_INVALID. Calls on one GPU are serialised by the library
+ * (the array keeps the last call's contents; nothing in it needs clearing).
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (HIP graph capture around the time loop, serving loops, harnesses that
+ * forbid hipMalloc). */
+int meso_hip_init_equilibrium_noalloc(meso_hip_ctx *ctx, const double *rho, const double *u3, const double *T);
 /* read-np-data restart: f = equilibrium (Shakhov: g_S/h_S with q) of the given
  * macros, which are taken as the cell state (cdugks_shakhov.cpp:81-101, cdugks.cpp:56-65). */
 int meso_hip_init_from_macros(meso_hip_ctx *ctx, const double *rho, const double *u3, const double *T,

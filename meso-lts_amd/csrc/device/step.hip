@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -3029,8 +3030,10 @@ int meso_hip_set_params(meso_hip_ctx *c, const meso_step_params *p) {
     return MESO_OK;
 }
 
+// scr (non-null: the allocation-free entry point) holds 5 doubles per cell for
+// the staged macros of init_equilibrium, which otherwise allocates them.
 static int init_common(meso_hip_ctx *c, const double *rho, const double *u3, const double *T, const double *q3,
-                       bool from_macros) {
+                       bool from_macros, double *scr = nullptr) {
     const bool shak = c->s.model == MESO_MODEL_CDUGKS_SHAKHOV;
     if (!rho || !u3 || (shak && !T) || (from_macros && shak && !q3))
         return set_err(c, MESO_E_INVALID, "null argument");
@@ -3052,7 +3055,13 @@ static int init_common(meso_hip_ctx *c, const double *rho, const double *u3, con
         }
         rc = dispatch<InitF>(c, a, (const double *)nullptr, (const double *)nullptr, (const double *)nullptr, true);
     } else {
-        if ((rc = dalloc(c, &tr, nc)) || (rc = dalloc(c, &tu, 3 * nc)) || (rc = dalloc(c, &tT, nc))) return rc;
+        if (scr) {
+            tr = scr;
+            tu = scr + nc;
+            tT = scr + 4 * nc;
+        } else if ((rc = dalloc(c, &tr, nc)) || (rc = dalloc(c, &tu, 3 * nc)) || (rc = dalloc(c, &tT, nc))) {
+            return rc;
+        }
         HIP_TRY(c, hipMemcpyAsync(tr, rho, nc * 8, hipMemcpyHostToDevice, c->stream));
         HIP_TRY(c, hipMemcpyAsync(tu, u3, 3 * nc * 8, hipMemcpyHostToDevice, c->stream));
         if (T) HIP_TRY(c, hipMemcpyAsync(tT, T, nc * 8, hipMemcpyHostToDevice, c->stream));
@@ -3060,9 +3069,9 @@ static int init_common(meso_hip_ctx *c, const double *rho, const double *u3, con
     }
     if (rc) return rc;
     HIP_TRY(c, hipStreamSynchronize(c->stream));
-    // temporaries are released right away
+    // temporaries are released right away (the scratch of the allocation-free entry point stays)
     for (double *t : {tr, tu, tT}) {
-        if (!t) continue;
+        if (!t || scr) continue;
         auto it = std::find(c->allocs.begin(), c->allocs.end(), (void *)t);
         if (it != c->allocs.end()) c->allocs.erase(it);
         (void)hipFree(t);
@@ -3077,6 +3086,50 @@ static int init_common(meso_hip_ctx *c, const double *rho, const double *u3, con
 int meso_hip_init_equilibrium(meso_hip_ctx *c, const double *rho, const double *u3, const double *T) {
     if (!c) return set_err(c, MESO_E_INVALID, "null context");
     return init_common(c, rho, u3, T, nullptr, false);
+}
+
+}  // extern "C"
+
+// ------------------------------------------- allocation-free initial state
+// Scratch of meso_hip_init_equilibrium_noalloc: the per-cell macros (rho,
+// u[3], T; 40 B per cell) that the initial-state kernel reads. The module holds
+// one copy per device for the life of the process. Host code reaches it only
+// through the address hipGetSymbolAddress returns for the current device,
+// kept per device in g_scratch_ptr; the array's name is not a device address
+// on the host.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;  // 16 MiB: up to 419,430 cells
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu[NOALLOC_MAX_DEVICES];
+
+extern "C" {
+
+int meso_hip_init_equilibrium_noalloc(meso_hip_ctx *c, const double *rho, const double *u3, const double *T) {
+    if (!c) return set_err(c, MESO_E_INVALID, "null context");
+    const size_t nc = size_t(c->s.nc);
+    if (nc * 5 * sizeof(double) > NOALLOC_SCRATCH_BYTES)
+        return set_err(c, MESO_E_INVALID,
+                       "meso_hip_init_equilibrium_noalloc: " + std::to_string(nc) + " cells need " +
+                           std::to_string(nc * 40) + " B of scratch; its limit is 16 MiB (" +
+                           std::to_string(NOALLOC_SCRATCH_BYTES / 40) + " cells)");
+    if (c->device < 0 || c->device >= NOALLOC_MAX_DEVICES)
+        return set_err(c, MESO_E_DEVICE, "meso_hip_init_equilibrium_noalloc: device index out of range");
+    // one call at a time per device: the call owns the device's scratch until
+    // its kernel has finished (init_common synchronises the stream)
+    std::lock_guard<std::mutex> lock(g_scratch_mu[c->device]);
+    if (!g_scratch_ptr[c->device]) {
+        int cur = -1;
+        HIP_TRY(c, hipGetDevice(&cur));
+        if (cur != c->device) HIP_TRY(c, hipSetDevice(c->device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != c->device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return set_err(c, MESO_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[c->device] = p;
+    }
+    return init_common(c, rho, u3, T, nullptr, false, static_cast<double *>(g_scratch_ptr[c->device]));
 }
 
 int meso_hip_init_from_macros(meso_hip_ctx *c, const double *rho, const double *u3, const double *T,

@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
             "meso_hip_last_error": ([vp], C.c_char_p),
             "meso_hip_init_equilibrium": ([vp, _d, _d, _d], C.c_int),
             "meso_hip_init_from_macros": ([vp, _d, _d, _d, _d], C.c_int),
+            "meso_hip_init_equilibrium_noalloc": ([vp, _d, _d, _d], C.c_int),
             "meso_hip_set_state": ([vp, _d, C.c_int32, _d, _d, _d, _d], C.c_int),
             "meso_hip_get_state": ([vp, _d, C.c_int32], C.c_int),
             "meso_hip_step": ([vp, C.c_int32, C.POINTER(StepReport)], C.c_int),
@@ -292,6 +293,13 @@ class Device:
         if rho is None:
             rho, u, T = self.case.initial_patch()
         self._check(lib().meso_hip_init_equilibrium(self._h, dptr(as_f64(rho)), dptr(as_f64(u)), dptr(as_f64(T))))
+
+    def init_equilibrium_noalloc(self, rho=None, u=None, T=None):
+        """init_equilibrium through meso_hip_init_equilibrium_noalloc (no device allocation)."""
+        if rho is None:
+            rho, u, T = self.case.initial_patch()
+        self._check(lib().meso_hip_init_equilibrium_noalloc(self._h, dptr(as_f64(rho)), dptr(as_f64(u)),
+                                                            dptr(as_f64(T))))
 
     def init_from_macros(self, rho, u, T, q):
         self._check(lib().meso_hip_init_from_macros(self._h, dptr(as_f64(rho)), dptr(as_f64(u)), dptr(as_f64(T)),
