@@ -411,8 +411,8 @@ __device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn,
 //     that side (tile = side * p_chunks + chunk; the slot of a side the patch
 //     does not own is written by the patch that owns it, a face with both
 //     sides owned puts its sum in slot 0 and zeros in slot 1).
-// XS: FP32X storage (S = double): rows and f-bar+ in HBM as FP32 word and
-// extension byte of the deviation from f_ref.
+// XS: FP32X storage (S = double): the cell rows in HBM as FP32 word and
+// extension byte of the deviation from f_ref (f-bar+ stays FP64).
 template <int MODEL, class S, int VT, int D, bool LIM, bool CELLS, bool XS = false>
 __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
@@ -608,17 +608,18 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                     // (no flux)
                     const size_t fi = size_t(fg) * nvp + v;
                     if (mode == meso_patch::SIDE_BOUNDARY || up) {
-                        const S fv = S(mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
-                                                                                 : fc[d] + Xd);
+                        const double fvd = mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
+                                                                                     : fc[d] + Xd;
+                        const S fv = S(fvd);
                         *dst = fv;
-                        if constexpr (XS) {
-                            float w;
-                            unsigned char e;
-                            x32_encode(double(fv) - ref[d], w, e);
-                            ((float *)(d == 0 ? a.fbar_g : a.fbar_h))[fi] = w;
-                            (d == 0 ? a.xbar_g : a.xbar_h)[fi] = e;
-                        } else {
-                            (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h)[fi] = fv;
+                        (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h)[fi] = fv;
+                        // FP32: the faces of the boundary kernel (those with
+                        // a boundary mark; an unmarked open face, own == nei,
+                        // has index -1 and is not one of them) also keep the
+                        // unrounded value for its wall sums and fluxes
+                        if (sizeof(S) == 4 && mode == meso_patch::SIDE_BOUNDARY) {
+                            const int bi = a.face_bnd[fg];
+                            if (bi >= 0) (d == 0 ? a.fbnd_g : a.fbnd_h)[size_t(bi) * nvp + v] = fvd;
                         }
                     } else if (dead) {
                         atomicAdd(dst, S(0.5 * (fc[d] + Xd)));
@@ -843,13 +844,8 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                     const int l = lf + k * FS < nfp ? lf + k * FS : lf;
                     const int fk = faces[l].face;
                     const size_t o = size_t(fk) * nvp + v;
-                    if constexpr (XS) {
-                        fbv[k][0] = S(x32_decode(((const float *)fbg)[o], a.xbar_g[o]) + ref[0]);
-                        fbv[k][1] = SH ? S(x32_decode(((const float *)fbh)[o], a.xbar_h[o]) + ref[1]) : S(0);
-                    } else {
-                        fbv[k][0] = fbg[o];
-                        fbv[k][1] = SH ? fbh[o] : S(0);
-                    }
+                    fbv[k][0] = fbg[o];
+                    fbv[k][1] = SH ? fbh[o] : S(0);
                     if constexpr (PF) {
                         const S *tb = ftab + size_t(fk) * a.nt;
                         tav[k][0] = tb[L.vt & 1023];

@@ -164,19 +164,19 @@ __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restri
     const int nvp = a.s.nvp;
     if constexpr (sizeof(T) == 8) {
         // FP64 and FP32X storage with the patch kernels: the moment kernel
-        // wrote this face's f-bar+ (the same reconstruction; FP32X keeps 31
-        // mantissa bits of its deviation from f_ref); FP32 storage rounds it,
-        // so it reconstructs here in FP64
+        // wrote this face's f-bar+ (the same reconstruction, FP64); FP32
+        // storage rounds it, so it reconstructs here in FP64
         if (a.fbar_g) {
-            const size_t i = size_t(f) * nvp + v;
-            double fbar;
-            if (a.xbar_g) {
-                const float *w = (const float *)(k == 0 ? a.fbar_g : a.fbar_h);
-                fbar = x32_decode(w[i], (k == 0 ? a.xbar_g : a.xbar_h)[i]) + a.fref_d[k * a.s.nv + v];
-            } else {
-                fbar = double((k == 0 ? (const T *)a.fbar_g : (const T *)a.fbar_h)[i]);
-            }
-            return (1.0 - mix) * fbar + mix * eq_full;
+            const T *fbr = k == 0 ? (const T *)a.fbar_g : (const T *)a.fbar_h;
+            return (1.0 - mix) * double(fbr[size_t(f) * nvp + v]) + mix * eq_full;
+        }
+    }
+    if constexpr (sizeof(T) == 4) {
+        // FP32 storage with the patch kernels: the moment kernel kept this
+        // boundary face's f-bar+ unrounded (FP64 deviation from f_ref)
+        if (a.fbnd_g) {
+            const double fb = (k == 0 ? a.fbnd_g : a.fbnd_h)[size_t(a.face_bnd[f]) * nvp + v] + a.fref_d[k * a.s.nv + v];
+            return (1.0 - mix) * fb + mix * eq_full;
         }
     }
     // (FP32X storage always runs the patch kernels, so its values come from f-bar+ above)
@@ -1743,6 +1743,7 @@ struct meso_hip_ctx {
         *mp_rows = nullptr;
     void *crec = nullptr, *cell_tab = nullptr;
     void *fbar[2] = {nullptr, nullptr};  // patch path: face values g, h [nf][nvp] (storage type)
+    double *fbnd[2] = {nullptr, nullptr};  // FP32 patch path: boundary-face values, FP64 [nb][nvp]
     int *slot_of = nullptr;  // meso's velocity index -> storage slot (null: identity)
     int *cf_ptr = nullptr, *cf_ids = nullptr, *nb_ptr = nullptr, *nb_ids = nullptr, *face_bnd = nullptr,
         *bnd_face = nullptr, *bnd_mark = nullptr, *mirror = nullptr;
@@ -1754,7 +1755,6 @@ struct meso_hip_ctx {
     // extension bytes, [buffer][distribution]
     bool ext = false;
     unsigned char *fx[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-    unsigned char *xbar[2] = {nullptr, nullptr};  // FP32X: extension bytes of f-bar+
     // masked cells (meso_hip_set_mask): 1 per cell of a group with `mask switch True`
     int *mask = nullptr;
     bool has_mask = false;
@@ -1923,14 +1923,14 @@ KArgs make_args(meso_hip_ctx *c) {
     a.fin_h = c->f[i][1];
     a.fout_g = c->f[o][0];
     a.fout_h = c->f[o][1];
-    a.xbar_g = c->xbar[0];
-    a.xbar_h = c->xbar[1];
     a.xin_g = c->fx[i][0];
     a.xin_h = c->fx[i][1];
     a.xout_g = c->fx[o][0];
     a.xout_h = c->fx[o][1];
     a.mask = c->has_mask ? c->mask : nullptr;
     a.masked = c->flags + 5;
+    a.fbnd_g = c->fbnd[0];
+    a.fbnd_h = c->fbnd[1];
     a.fbar_g = c->fbar[0];
     a.fbar_h = c->fbar[1];
     a.rho_in = c->rho[i];
@@ -2971,9 +2971,11 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         }
         for (int k = 0; k < c->nd; ++k) {
             char *fb = nullptr;
-            if ((rc = dalloc(c, &fb, size_t(nf) * c->s.nvp * fes))) return fail(rc);
+            if ((rc = dalloc(c, &fb, size_t(nf) * c->s.nvp * es))) return fail(rc);
             c->fbar[k] = fb;
-            if (ext && (rc = dalloc(c, &c->xbar[k], size_t(nf) * c->s.nvp))) return fail(rc);
+            if (storage == MESO_STORAGE_FP32 && c->s.nb > 0 &&
+                (rc = dalloc(c, &c->fbnd[k], size_t(c->s.nb) * c->s.nvp)))
+                return fail(rc);
         }
     }
     c->xi_h = xi_s;

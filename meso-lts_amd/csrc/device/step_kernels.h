@@ -136,7 +136,9 @@ struct KArgs {
     // patch kernels: f-bar+ of every face, written by the moment kernel and
     // read by the update kernel (face-major [nf][nvp], storage type)
     void *fbar_g, *fbar_h;
-    unsigned char *xbar_g, *xbar_h;  // FP32X: extension bytes of f-bar+ (null otherwise)
+    // FP32 storage: f-bar+ of the boundary faces unrounded (FP64 deviation
+    // from f_ref), [boundary face][nvp], for k_boundary; null otherwise
+    double *fbnd_g, *fbnd_h;
     void *cell_tab;             // [nc][2][nt] axis tables of the cell equilibria
     // FP32 storage keeps f - f_ref; f_ref is the equilibrium `ref` at rest.
     EqParams ref;

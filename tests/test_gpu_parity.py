@@ -4,7 +4,10 @@ the same seeded inputs.
 Tolerances (relative, against the oracle):
   FP64 storage: max-norm 1e-10 on rho, u, T, q and flux moments after 10-20
   steps (summation order of the velocity reductions differs from meso's).
-  FP32 storage: relative L2 <= 2e-6 on every macro component after 200 steps.
+  FP32 storage: relative L2 <= 2e-6 on every macro component after 25-200
+  steps (channels with a small cross-flow: 2e-5, see FP32_TOL_CASE). FP32
+  storage does not hold 1e-6 over long runs (it stagnates near a steady state);
+  FP32X storage, which does, is tested in tests/test_gpu_precision.py.
 """
 import numpy as np
 import pytest
@@ -66,7 +69,18 @@ def test_fp64_matches_oracle(case_factory, name):
 # T < 0 after ~30 steps (the oracle and the reference do the same), and meso's
 # four-state Riemann problem (2990 unstructured velocities, no axis tables), 50.
 FP32_CASES = {"bgk_quad8": 200, "shak_quad6": 200, "shak_tri6": 25, "shak_hex3": 200, "shak_quad12_nc24": 200,
-              "shak_quad6_lim": 200, "shak_hex3_lim": 200, "neumann": 50, "kn0075_cavity": 200}
+              "shak_quad6_lim": 200, "shak_hex3_lim": 200, "neumann": 50, "kn0075_cavity": 200,
+              # the other boundary kinds in FP32 storage: farfield and pressure
+              # inlets/outlets, symmetry and slip walls (mirror-boundary cells)
+              "shak_channel_pressure": 200, "shak_channel_farfield": 200, "bgk_channel_farfield": 200,
+              "shak_sym_slip": 200, "shak_sym_slip_lim": 200}
+# The channels carry a small cross-flow V (and heat flux) against a large
+# stream: after 200 steps the FP32 storage rounding gives 2.5e-6 (farfield V),
+# 4.0e-6 (pressure qx), 6.2e-6 and 7.3e-6 (symmetry / slip V, without and with
+# the limiter), measured; a wrong sign or term in a boundary gives 1e-3 or
+# more. FP32X storage holds these cases to 1e-7 (tests/test_gpu_precision.py).
+FP32_TOL_CASE = {"shak_channel_pressure": 2e-5, "shak_channel_farfield": 2e-5, "shak_sym_slip": 2e-5,
+                 "shak_sym_slip_lim": 2e-5}
 
 
 @pytest.mark.parametrize("name", list(FP32_CASES))
@@ -74,15 +88,18 @@ def test_fp32_storage_close_to_oracle(case_factory, name):
     """FP32 storage of the distributions against the FP64 oracle after many steps."""
     cs, dev, orc, out = run_pair(case_factory(name), [FP32_CASES[name]], storage=capi.STORAGE_FP32)
     s, md, mo, _, _ = out[-1]
+    _say(f"[gpu] fp32 {name}: " + " ".join(
+        f"{k}={rel_l2(md[k].ravel(), mo[k].ravel()):.2e}" for k in keys_of(cs)))
+    tol = FP32_TOL_CASE.get(name, FP32_TOL)
     for k in keys_of(cs):
         if k == "q" or k == "u":
             comps = [0, 1, 2] if cs.mesh.dim == 3 else [0, 1]
             for i in comps:
                 err = rel_l2(md[k][:, i], mo[k][:, i])
-                assert err < FP32_TOL, f"{name} {k}[{i}]: {err:.2e}"
+                assert err < tol, f"{name} {k}[{i}]: {err:.2e}"
         else:
             err = rel_l2(md[k], mo[k])
-            assert err < FP32_TOL, f"{name} {k}: {err:.2e}"
+            assert err < tol, f"{name} {k}: {err:.2e}"
 
 
 def test_state_roundtrip_and_restart(case_factory):

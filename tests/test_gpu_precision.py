@@ -44,7 +44,7 @@ def macro_errors(cs, md, mo):
 # pressure boundaries, 2D and 3D, with and without the limiter
 FP32X_CASES = {"shak_quad6": 200, "shak_hex3": 200, "shak_quad12_nc24": 200, "shak_kn1_quad10": 200,
                "shak_quad6_lim": 200, "shak_channel_pressure": 200, "shak_channel_farfield": 200,
-               "kn0075_cavity": 100}
+               "shak_hex3_lim": 200, "kn0075_cavity": 100}
 
 
 @pytest.mark.parametrize("name", list(FP32X_CASES))
