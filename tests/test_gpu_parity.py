@@ -5,7 +5,8 @@ Tolerances (relative, against the oracle):
   FP64 storage: max-norm 1e-10 on rho, u, T, q and flux moments after 10-20
   steps (summation order of the velocity reductions differs from meso's).
   FP32 storage: relative L2 <= 2e-6 on every macro component after 25-200
-  steps (channels with a small cross-flow: 2e-5, see FP32_TOL_CASE). FP32
+  steps (channels with a small cross-flow: 5x the FP32 storage-rounding floor
+  of the same case, storage_floor). FP32
   storage does not hold 1e-6 over long runs (it stagnates near a steady state);
   FP32X storage, which does, is tested in tests/test_gpu_precision.py.
 """
@@ -75,12 +76,52 @@ FP32_CASES = {"bgk_quad8": 200, "shak_quad6": 200, "shak_tri6": 25, "shak_hex3":
               "shak_channel_pressure": 200, "shak_channel_farfield": 200, "bgk_channel_farfield": 200,
               "shak_sym_slip": 200, "shak_sym_slip_lim": 200}
 # The channels carry a small cross-flow V (and heat flux) against a large
-# stream: after 200 steps the FP32 storage rounding gives 2.5e-6 (farfield V),
-# 4.0e-6 (pressure qx), 6.2e-6 and 7.3e-6 (symmetry / slip V, without and with
-# the limiter), measured; a wrong sign or term in a boundary gives 1e-3 or
-# more. FP32X storage holds these cases to 1e-7 (tests/test_gpu_precision.py).
-FP32_TOL_CASE = {"shak_channel_pressure": 2e-5, "shak_channel_farfield": 2e-5, "shak_sym_slip": 2e-5,
-                 "shak_sym_slip_lim": 2e-5}
+# stream, so their small components see the FP32 rounding of the stored state
+# at a larger relative size. Their tolerance is taken from the same case run
+# on the CPU with that rounding alone (storage_floor: the FP64 oracle, the
+# stored deviation from f_ref rounded to FP32 after every step): the GPU must
+# stay within 5x that floor per component (at least FP32_TOL). On the symmetry
+# and slip channel the floor is qy 3.0e-5 after 200 steps (the GPU: 3.5e-5);
+# a wrong sign or term in a boundary gives 1e-3 or more. FP32X storage holds
+# these cases to 1e-7 (tests/test_gpu_precision.py).
+FLOOR_CASES = ("shak_channel_pressure", "shak_channel_farfield", "shak_sym_slip", "shak_sym_slip_lim")
+
+
+def f_ref_of(cs, orc):
+    """The reference equilibrium of the FP32 deviation storage: the rest
+    Maxwellian pair (Shakhov) or the constant rho_ref (BGK) at the mean
+    initial density and temperature, per velocity ([2, nv, 1])."""
+    m = orc.macros()
+    p = cs.params
+    xi = cs.arrays()["xi"]
+    rho = m["rho"].mean()
+    if cs.model != capi.MODEL_CDUGKS_SHAKHOV:
+        g = np.full(xi.shape[0], rho)
+        return np.stack([g, np.zeros_like(g)])[:, :, None]
+    RT = p.R * m["T"].mean()
+    D = cs.mesh.dim
+    A = rho * (0.5 / RT) / np.pi if D == 2 else rho / (2.0 * np.pi * RT) ** 1.5
+    g = A * np.exp(-(0.5 / RT) * (xi ** 2).sum(1))
+    return np.stack([g, (p.K + 3.0 - D) * RT * g])[:, :, None]
+
+
+def storage_floor(cs, steps):
+    """Relative L2 of every macro component when only the stored state is
+    rounded to FP32 (deviation from f_ref, after every step), against the
+    unrounded oracle run."""
+    a, b = Oracle(cs), Oracle(cs)
+    a.init_equilibrium()
+    b.init_equilibrium()
+    ref = f_ref_of(cs, b)
+    for _ in range(steps):
+        a.step(1)
+        b.step(1)
+        f = ((b.get_state() - ref).astype(np.float32).astype(np.float64)) + ref
+        m = b.macros()
+        b.set_state(f, m["rho"], m["u"], m["T"], m["q"])
+    ma, mb = a.macros(), b.macros()
+    return {k: [rel_l2(mb[k][:, i], ma[k][:, i]) for i in range(cs.mesh.dim)] if ma[k].ndim == 2
+            else rel_l2(mb[k], ma[k]) for k in keys_of(cs)}
 
 
 @pytest.mark.parametrize("name", list(FP32_CASES))
@@ -90,16 +131,19 @@ def test_fp32_storage_close_to_oracle(case_factory, name):
     s, md, mo, _, _ = out[-1]
     _say(f"[gpu] fp32 {name}: " + " ".join(
         f"{k}={rel_l2(md[k].ravel(), mo[k].ravel()):.2e}" for k in keys_of(cs)))
-    tol = FP32_TOL_CASE.get(name, FP32_TOL)
+    floor = storage_floor(cs, FP32_CASES[name]) if name in FLOOR_CASES else None
+    if floor is not None:
+        _say(f"[gpu] fp32 {name} storage-rounding floor: {floor}")
     for k in keys_of(cs):
         if k == "q" or k == "u":
-            comps = [0, 1, 2] if cs.mesh.dim == 3 else [0, 1]
-            for i in comps:
+            for i in range(cs.mesh.dim):
                 err = rel_l2(md[k][:, i], mo[k][:, i])
-                assert err < tol, f"{name} {k}[{i}]: {err:.2e}"
+                tol = FP32_TOL if floor is None else max(FP32_TOL, 5.0 * floor[k][i])
+                assert err < tol, f"{name} {k}[{i}]: {err:.2e} (tolerance {tol:.2e})"
         else:
             err = rel_l2(md[k], mo[k])
-            assert err < tol, f"{name} {k}: {err:.2e}"
+            tol = FP32_TOL if floor is None else max(FP32_TOL, 5.0 * floor[k])
+            assert err < tol, f"{name} {k}: {err:.2e} (tolerance {tol:.2e})"
 
 
 def test_state_roundtrip_and_restart(case_factory):
