@@ -125,9 +125,12 @@ def copy_calibration(mib):
 def batch_main(args):
     """The batched cavity sweep: N independent Shakhov cavities (quads 32^2 to
     128^2, Kn log-uniform in [0.01, 10], lid speed U[0.05, 0.3], Newton-Cotes
-    16^2 to 64^2 velocities, seeds 1000 + i), FP32 storage, advanced together
-    with meso_hip_step_batch; value = sum of NC * NV * steps / seconds. Also
-    timed: the same instances stepped one after the other (meso_hip_step)."""
+    16^2 to 64^2 velocities, seeds 1000 + i), FP32 storage; value = sum of
+    NC * NV * steps / seconds of the fused batch (meso_hip_batch_step: one
+    launch per stage over all instances, steps replayed as captured pairs
+    with --graphs). Also timed on the same devices: round-robin issue on the
+    instances' own streams (meso_hip_step_batch) and one instance after the
+    other (meso_hip_step)."""
     work = os.path.join(tempfile.gettempdir(), "meso_hip_bench")
     insts = configs.batch_instances(args.batch)
     cases = [capi.Case(configs.build_instance(i, work, args.scale)) for i in insts]
@@ -137,38 +140,62 @@ def batch_main(args):
         d = capi.Device(cs, capi.STORAGE_FP32)
         d.init_equilibrium()
         chunks.append(d.autotune(3) if args.autotune else 0)
-        d.set_graphs(args.graphs)
         devs.append(d)
     dvc = sum(cs.n_cells * cs.n_vel for cs in cases)
     log(f"[bench] batch of {len(cases)}: {dvc:.4e} DV-cells in total "
         f"(cells {min(c.n_cells for c in cases)}..{max(c.n_cells for c in cases)}, "
         f"velocities {min(c.n_vel for c in cases)}..{max(c.n_vel for c in cases)})")
-    capi.step_batch(devs, args.warmup)
+
+    def timed(run):
+        run(args.warmup)
+        for d in devs:
+            d.sync()
+        t0 = time.perf_counter()
+        reps = run(args.steps)
+        for d in devs:
+            d.sync()
+        secs = time.perf_counter() - t0
+        if reps is not None and any(r.bad_cells for r in reps):
+            raise RuntimeError("numerical failure in the batch")
+        return secs
+
+    batch = capi.Batch(devs)
+    batch.set_graphs(args.graphs)
+    batch.set_streams(args.batch_streams)
+    fused_info = batch.info()
+    tl = [d.tiles() for d in devs]
+    log("[bench] tiles after the batch plan: vchunk " + " ".join(str(t["vchunk"]) for t in tl))
+    log("[bench] LDS per block: moments " + " ".join(str(t["moment_lds"]) for t in tl) +
+        " | update " + " ".join(str(t["update_lds"]) for t in tl))
+    secs = timed(batch.step)
+    log(f"[bench] fused batch ({fused_info[0]} instances in {fused_info[1]} group(s), {fused_info[2]} launches "
+        f"per step): {secs / args.steps * 1e3:.3f} ms per step of all instances, {dvc * args.steps / secs:.4e} "
+        f"DV-cell updates/s")
+    batch.close()
     for d in devs:
-        d.sync()
-    t0 = time.perf_counter()
-    reps = capi.step_batch(devs, args.steps)
-    for d in devs:
-        d.sync()
-    secs = time.perf_counter() - t0
-    if any(r.bad_cells for r in reps):
-        raise RuntimeError("numerical failure in the batch")
-    t0 = time.perf_counter()
-    for d in devs:
-        d.step(args.steps)
-    seq = time.perf_counter() - t0
+        d.set_graphs(args.graphs)
+    rr = timed(lambda n: capi.step_batch(devs, n))
+    log(f"[bench] round-robin (meso_hip_step_batch): {rr / args.steps * 1e3:.3f} ms per step")
+
+    def one_by_one(n):
+        for d in devs:
+            d.step(n)
+    seq = timed(one_by_one)
+    log(f"[bench] one after the other: {seq / args.steps * 1e3:.3f} ms per step")
     ups = dvc * args.steps / secs
-    log(f"[bench] batch: {secs / args.steps * 1e3:.3f} ms per step of all instances, {ups:.4e} DV-cell updates/s; "
-        f"one after the other: {seq / args.steps * 1e3:.3f} ms")
     line = {
         "metric": METRIC, "value": ups, "unit": "DV-cell updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": secs / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"batch_{len(cases)}", "instances": len(cases), "dv_cells": dvc,
                    "autotune": bool(args.autotune), "graphs": bool(args.graphs),
-                   "storage": "fp32 g,h; fp64 moments/macros"},
+                   "storage": "fp32 g,h; fp64 moments/macros",
+                   "fused": {"instances": fused_info[0], "groups": fused_info[1],
+                             "launches_per_step": fused_info[2], "streams": args.batch_streams}},
+        "round_robin_ms_per_step": rr / args.steps * 1e3,
         "sequential_ms_per_step": seq / args.steps * 1e3,
-        "batch_over_sequential": seq / secs,
+        "fused_over_round_robin": rr / secs,
+        "fused_over_sequential": seq / secs,
         "vchunks": chunks if args.autotune else None,
     }
     for d in devs:
@@ -296,6 +323,8 @@ def main():
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="instead: the batched cavity sweep of N seeded instances (FP32 storage)")
+    ap.add_argument("--batch-streams", type=int, default=4, metavar="S",
+                    help="--batch: sub-batches of the fused batch, one stream each (meso_hip_batch_set_streams)")
     ap.add_argument("--autotune", action="store_true",
                     help="run meso_hip_autotune before timing (off by default: the default tiles are "
                          "deterministic, so that profiles of the same command see the same kernels)")

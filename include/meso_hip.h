@@ -279,6 +279,8 @@ typedef struct meso_tile_config {
     int32_t update_owned;     /* cells whose update the dominant update launch owns
                                  (patch kernels: every cell without a mirror boundary;
                                  per-cell kernels: the cells without a boundary face) */
+    int32_t moment_lds;       /* LDS bytes per block of the moment kernel (patch kernels) */
+    int32_t update_lds;       /* LDS bytes per block of the update kernel (patch kernels) */
 } meso_tile_config;
 int meso_hip_tiles(const meso_hip_ctx *ctx, meso_tile_config *out);
 /* enable != 0: meso_hip_step and meso_hip_step_batch run steps in pairs as
@@ -287,6 +289,48 @@ int meso_hip_tiles(const meso_hip_ctx *ctx, meso_tile_config *out);
  * never while kernel profiling is on). Results are those of the eager steps.
  * No counterpart in the reference. */
 int meso_hip_set_graphs(meso_hip_ctx *ctx, int32_t enable);
+/* Velocities per block of the patch kernels (a positive multiple of 64; 0:
+ * the default, enough chunks for 16 blocks per CU). Changes only the order of
+ * the velocity partial sums. MESO_E_INVALID when the partial-sum buffers do
+ * not hold that many chunks, or (for vchunk != 0) when the case runs the
+ * per-cell kernels. The autotuner's third stage, for callers that choose it
+ * themselves. No counterpart in the reference. */
+int meso_hip_set_vchunk(meso_hip_ctx *ctx, int32_t vchunk);
+
+/* A fixed set of contexts advanced with one launch per stage (the batched
+ * cavity sweep in one grid): the contexts whose step runs the patch kernels
+ * (large velocity sets, no mirror-boundary cells, no masked groups) are
+ * split into sub-batches (meso_hip_batch_set_streams) and grouped by kernel
+ * instance, and every stage of a step is one launch over the blocks of all
+ * contexts of a group, each block reading its context's arguments from a
+ * device table; every XCD takes a contiguous share of every context's
+ * blocks. Other contexts take their own launches in the same call.
+ * The plan (tables, block maps) is built on the first step, or by
+ * meso_hip_batch_info, and again after any context's parameters, state,
+ * tiles or buffer parity changed outside the batch. Building it sets each
+ * fused context's velocity chunk for the batch grid (fewer, larger blocks
+ * than a context alone takes: its share of the batch's DV-cells decides;
+ * meso_hip_tiles reports it, meso_hip_batch_set_rechunk(b, 0) keeps the
+ * contexts' own). Results are bit-identical to meso_hip_step on each context
+ * with the same tiles. meso_hip_batch_set_graphs(b, 1) replays two fused steps as one
+ * captured hipGraph. reports[k] is what meso_hip_step would report for
+ * ctxs[k]. The contexts must outlive the batch, share one device, and may not
+ * be stepped from another thread while the batch steps. No counterpart in the
+ * reference, which runs one case per process (src/meso/exec.cpp:141-165). */
+typedef struct meso_hip_batch meso_hip_batch;
+int meso_hip_batch_create(meso_hip_ctx *const *ctxs, int32_t n, meso_hip_batch **batch);
+int meso_hip_batch_step(meso_hip_batch *batch, int32_t nsteps, meso_step_report *reports);
+int meso_hip_batch_set_graphs(meso_hip_batch *batch, int32_t enable);
+int meso_hip_batch_set_rechunk(meso_hip_batch *batch, int32_t enable);
+/* Sub-batches (1..16, default 4: the hardware queues a process gets): the
+ * fused contexts are split by DV-cells over n streams, each launching its
+ * own share of every stage, so that the small stages of one sub-batch overlap
+ * the large ones of another. */
+int meso_hip_batch_set_streams(meso_hip_batch *batch, int32_t n);
+/* fused: contexts in the fused launches; groups: kernel instances;
+ * launches_per_step: launches of one batch step (fused part) */
+int meso_hip_batch_info(meso_hip_batch *batch, int32_t *fused, int32_t *groups, int32_t *launches_per_step);
+void meso_hip_batch_destroy(meso_hip_batch *batch);
 
 /* Cell macros after the last step: rho[NC], u3[NC*3], T[NC], q3[NC*3] (NULL skips). */
 int meso_hip_get_macros(meso_hip_ctx *ctx, double *rho, double *u3, double *T, double *q3);

@@ -61,12 +61,13 @@ struct CellRec<double, D> {
 // ------------------------------------------------------------------- prep
 // Per cell of the update patches: conservative macro update (as k_cell_update
 // does for its cells), the constants of the f update, the cell's axis tables.
+// bid: the block's index in its context's own grid.
 template <int MODEL, class S, bool TAB, int D>
-__global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
+__device__ __forceinline__ void d_cell_prep(const KArgs &a, int bid) {
     if (*a.halt) return;
     constexpr int WPB = BLOCK / 64;
     const int lane = threadIdx.x % 64;
-    const int li = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + int(threadIdx.x / 64));
+    const int li = __builtin_amdgcn_readfirstlane(bid * WPB + int(threadIdx.x / 64));
     if (li >= a.n_int_cells) return;
     const int c = a.cell_list[li];
     CellStep<MODEL, D> cs;
@@ -101,15 +102,25 @@ __global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
         r.Cs = cs.Cs;
         r.dtv = cs.dt_v;
         for (int i = 0; i < D; ++i) r.u[i] = cs.u[i];
-        reinterpret_cast<CellRec<S, D> *>(a.crec)[c] = r;
+        static_cast<CellRec<S, D> *>(a.crec)[c] = r;
     }
     if (TAB) {
-        S *dst = reinterpret_cast<S *>(a.cell_tab) + size_t(c) * 2 * a.nt;
+        S *dst = static_cast<S *>(a.cell_tab) + size_t(c) * 2 * a.nt;
         for (int t = lane; t < a.nt; t += 64) {
             dst[t] = tab_entry<S>(a, cs.en, t);
             dst[a.nt + t] = tab_entry<S>(a, cs.e, t);
         }
     }
+}
+template <int MODEL, class S, bool TAB, int D>
+__global__ __launch_bounds__(BLOCK) void k_cell_prep(KArgs a) {
+    d_cell_prep<MODEL, S, TAB, D>(a, blockIdx.x);
+}
+template <int MODEL, class S, bool TAB, int D>
+__global__ __launch_bounds__(BLOCK) void kb_cell_prep(const KArgs *__restrict__ tab, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;  // padding of a short XCD list
+    d_cell_prep<MODEL, S, TAB, D>(batch_args(tab, m.x), m.y);
 }
 
 // LSQ gradient g[0..D-1] of one item from the staged rows of distribution d:
@@ -158,9 +169,10 @@ __device__ __forceinline__ void stage_values_x(const KArgs &a, const int *rows, 
     constexpr int NSLOT = (BLOCKP / 64) * (64 / VT);
     constexpr int B = 8;
     const float *wg = (const float *)a.fin_g, *wh = (const float *)a.fin_h;
+    const unsigned *xg = a.xin_g.as<const unsigned>(), *xh = a.xin_h.as<const unsigned>();
     for (int base = slot; base < nr; base += NSLOT * B) {
         float w[B][ND];
-        unsigned char e[B][ND];
+        unsigned e[B][ND];
 #pragma unroll
         for (int k = 0; k < B; ++k) {
             const int r = base + k * NSLOT;
@@ -169,7 +181,11 @@ __device__ __forceinline__ void stage_values_x(const KArgs &a, const int *rows, 
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 w[k][d] = on ? (d == 0 ? wg : wh)[o] : 0.0f;
-                e[k][d] = on ? (d == 0 ? a.xin_g : a.xin_h)[o] : 0;
+                // the extension byte from the dword holding it (rows start
+                // on 64-byte boundaries): dword loads shared by four lanes
+                // took the FP32X moment kernel from 10.7 to 9.8 ms against
+                // byte loads
+                e[k][d] = on ? (d == 0 ? xg : xh)[o >> 2] >> (8 * (o & 3)) : 0u;
             }
         }
 #pragma unroll
@@ -342,7 +358,7 @@ __device__ __forceinline__ void face_eq_rec(const KArgs &a, const typename Updat
             if constexpr (TAB && PF)
                 shakhov_tab_vals_f(e, tav[0], tav[1], tav[2], L.xf, D, a.s.K, L.gref, g, h);
             else if (TAB)
-                shakhov_tab_f(e, reinterpret_cast<const float *>(a.face_tab) + size_t(f) * a.nt, L.vt, L.xf, D, a.s.K,
+                shakhov_tab_f(e, static_cast<const float *>(a.face_tab) + size_t(f) * a.nt, L.vt, L.xf, D, a.s.K,
                               L.gref, g, h);
             else
                 shakhov_devf(e, L.xf, D, a.s.K, L.gref, g, h);
@@ -357,7 +373,7 @@ __device__ __forceinline__ void face_eq_rec(const KArgs &a, const typename Updat
             if constexpr (TAB && PF)
                 shakhov_tab_vals_d(e, tav[0], tav[1], tav[2], x, D, a.s.K, gs, hs);
             else if (TAB)
-                shakhov_tab_d(e, reinterpret_cast<const double *>(a.face_tab) + size_t(f) * a.nt, L.vt, x, D, a.s.K,
+                shakhov_tab_d(e, static_cast<const double *>(a.face_tab) + size_t(f) * a.nt, L.vt, x, D, a.s.K,
                               gs, hs);
             else
                 shakhov_eq<double>(e, x, D, a.s.K, gs, hs);
@@ -413,11 +429,12 @@ __device__ __forceinline__ void bnd_value_lane(const KArgs &a, int f, double sn,
 //     sides owned puts its sum in slot 0 and zeros in slot 1).
 // XS: FP32X storage (S = double): the cell rows in HBM as FP32 word and
 // extension byte of the deviation from f_ref (f-bar+ stays FP64).
+// b: the block's logical index in its context's grid (XCD-mapped, xcd_block).
 template <int MODEL, class S, int VT, int D, bool LIM, bool CELLS, bool XS = false>
-__global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
+__device__ __forceinline__ void d_patch_moments(const KArgs &a, const int b) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.bad_count > 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) *a.halt = 1;
+        if (b == 0 && threadIdx.x == 0) *a.halt = 1;
         return;
     }
     constexpr bool SH = MODEL == CDUGKS_SHAKHOV;
@@ -429,10 +446,9 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
     constexpr int VTP = VT + 1;  // odd row stride: conflict-free reads with lane = face
     using Item = meso_patch::ItemT<D>;
     using Face = meso_patch::FaceT<D>;
-    const Item *mp_items = reinterpret_cast<const Item *>(a.mp_items);
-    const Face *mp_faces = reinterpret_cast<const Face *>(a.mp_faces);
+    const Item *mp_items = static_cast<const Item *>(a.mp_items);
+    const Face *mp_faces = static_cast<const Face *>(a.mp_faces);
     const int np = a.n_mp;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
     const int chunk = b / np, p = b - chunk * np;
     const int i0 = a.mp_item_ptr[p], ni = a.mp_item_ptr[p + 1] - i0;
     const int f0 = a.mp_face_ptr[p], nfp = a.mp_face_ptr[p + 1] - f0;
@@ -608,19 +624,15 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
                     // (no flux)
                     const size_t fi = size_t(fg) * nvp + v;
                     if (mode == meso_patch::SIDE_BOUNDARY || up) {
-                        const double fvd = mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
-                                                                                     : fc[d] + Xd;
-                        const S fv = S(fvd);
+                        // (FP32 storage: the boundary kernel reconstructs
+                        // its faces' values again in FP64 rather than reading
+                        // these rounded ones; keeping unrounded copies here
+                        // took the 2D kernel from 69 to 96 VGPRs, 3 to 2
+                        // blocks per CU, 4.9 to 6.9 ms)
+                        const S fv = S(mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
+                                                                                 : fc[d] + Xd);
                         *dst = fv;
                         (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h)[fi] = fv;
-                        // FP32: the faces of the boundary kernel (those with
-                        // a boundary mark; an unmarked open face, own == nei,
-                        // has index -1 and is not one of them) also keep the
-                        // unrounded value for its wall sums and fluxes
-                        if (sizeof(S) == 4 && mode == meso_patch::SIDE_BOUNDARY) {
-                            const int bi = a.face_bnd[fg];
-                            if (bi >= 0) (d == 0 ? a.fbnd_g : a.fbnd_h)[size_t(bi) * nvp + v] = fvd;
-                        }
                     } else if (dead) {
                         atomicAdd(dst, S(0.5 * (fc[d] + Xd)));
                     }
@@ -706,6 +718,17 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
         }
     }
 }
+template <int MODEL, class S, int VT, int D, bool LIM, bool CELLS, bool XS = false>
+__global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
+    d_patch_moments<MODEL, S, VT, D, LIM, CELLS, XS>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+template <int MODEL, class S, int VT, int D, bool LIM, bool CELLS, bool XS = false>
+__global__ __launch_bounds__(BLOCKP, 4) void kb_patch_moments(const KArgs *__restrict__ tab,
+                                                               const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;  // padding of a short XCD list
+    d_patch_moments<MODEL, S, VT, D, LIM, CELLS, XS>(batch_args(tab, m.x), m.y);
+}
 
 // ------------------------------------------------------------- update
 // Cells of the update patches: face fluxes from the f-bar+ rows the moment
@@ -716,7 +739,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_moments(KArgs a) {
 // would cost address arithmetic and L2 latency). LIM only selects the
 // instance that pairs with the limited moment kernel.
 template <int MODEL, class S, int VT, bool TAB, int D, bool LIM, bool XS = false>
-__global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
+__device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     if (*a.halt) return;
     constexpr bool SH = MODEL == CDUGKS_SHAKHOV;
@@ -726,10 +749,9 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
     constexpr int NSLOT = NW * IPW;
     using Item = meso_patch::ItemT<D>;
     using Face = meso_patch::FaceT<D>;
-    const Item *up_items = reinterpret_cast<const Item *>(a.up_items);
-    const Face *up_faces = reinterpret_cast<const Face *>(a.up_faces);
+    const Item *up_items = static_cast<const Item *>(a.up_items);
+    const Face *up_faces = static_cast<const Face *>(a.up_faces);
     const int np = a.n_up;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
     const int chunk = b / np, p = b - chunk * np;
     const int i0 = a.up_item_ptr[p];
     const int f0 = a.up_face_ptr[p], nfp = a.up_face_ptr[p + 1] - f0;
@@ -769,7 +791,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
     }
     for (int r = threadIdx.x; r < nown; r += BLOCKP) {
         const int c = up_items[i0 + r].cell;
-        crl[r] = reinterpret_cast<const CellRec<S, D> *>(a.crec)[c];
+        crl[r] = static_cast<const CellRec<S, D> *>(a.crec)[c];
         if constexpr (sizeof(S) == 4) csl[r] = cell_s<D>(a.cell_S + 3 * c);
     }
     const int ntiles = (a.s.nv + VT - 1) / VT;
@@ -835,7 +857,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             // axis-table entries) issued before any is used (more loads in
             // flight per wave; 2 and 8 were slower); FP64 one (two were slower)
             constexpr int NB = sizeof(S) == 4 ? 4 : 1;
-            const S *ftab = reinterpret_cast<const S *>(a.face_tab);
+            const S *ftab = static_cast<const S *>(a.face_tab);
             for (int lf = threadIdx.x / VT; lf < nfp; lf += NB * FS) {
                 S fbv[NB][2];
                 S tav[NB][3];
@@ -874,7 +896,7 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
             // latency overlaps the flux sums
             S ta[2][3];
             if constexpr (TAB && SH) {
-                const S *tb = reinterpret_cast<const S *>(a.cell_tab) + size_t(c) * 2 * a.nt;
+                const S *tb = static_cast<const S *>(a.cell_tab) + size_t(c) * 2 * a.nt;
 #pragma unroll
                 for (int q = 0; q < 2; ++q) {
                     ta[q][0] = tb[q * a.nt + (L.vt & 1023)];
@@ -1028,4 +1050,15 @@ __global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
                 for (int i = 0; i < 4; ++i) dst[i] = i < D ? qv[i] : 0.0;
             }
         }
+}
+template <int MODEL, class S, int VT, bool TAB, int D, bool LIM, bool XS = false>
+__global__ __launch_bounds__(BLOCKP, 4) void k_patch_update(KArgs a) {
+    d_patch_update<MODEL, S, VT, TAB, D, LIM, XS>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+template <int MODEL, class S, int VT, bool TAB, int D, bool LIM, bool XS = false>
+__global__ __launch_bounds__(BLOCKP, 4) void kb_patch_update(const KArgs *__restrict__ tab,
+                                                              const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;  // padding of a short XCD list
+    d_patch_update<MODEL, S, VT, TAB, D, LIM, XS>(batch_args(tab, m.x), m.y);
 }

@@ -23,7 +23,7 @@ constexpr int BLOCK = 256;
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ const meso_mark_desc &mark_of(const KArgs &a, int m) {
-    return reinterpret_cast<const meso_mark_desc *>(a.marks)[m];
+    return static_cast<const meso_mark_desc *>(a.marks)[m];
 }
 
 // Compressible::solve_pressure (pressure_boundary.cpp:15-18)
@@ -171,14 +171,6 @@ __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restri
             return (1.0 - mix) * double(fbr[size_t(f) * nvp + v]) + mix * eq_full;
         }
     }
-    if constexpr (sizeof(T) == 4) {
-        // FP32 storage with the patch kernels: the moment kernel kept this
-        // boundary face's f-bar+ unrounded (FP64 deviation from f_ref)
-        if (a.fbnd_g) {
-            const double fb = (k == 0 ? a.fbnd_g : a.fbnd_h)[size_t(a.face_bnd[f]) * nvp + v] + a.fref_d[k * a.s.nv + v];
-            return (1.0 - mix) * fb + mix * eq_full;
-        }
-    }
     // (FP32X storage always runs the patch kernels, so its values come from f-bar+ above)
     auto ld = [&](size_t i) -> double { return double(fd[i]); };
     const double fs = ld(size_t(c) * nvp + v);
@@ -289,7 +281,7 @@ __device__ __forceinline__ void bnd_value_full(const KArgs &a, const BndParams &
 template <int MODEL>
 __device__ __forceinline__ void flag_bad(const KArgs &a, int c) {
     if (MODEL == CDUGKS_SHAKHOV && a.mask && a.mask[c]) {
-        const_cast<int *>(a.mask)[c] = 2;
+        const_cast<int *>(static_cast<const int *>(a.mask))[c] = 2;
         atomicAdd(a.masked, 1);
         return;
     }
@@ -389,10 +381,11 @@ __device__ __forceinline__ double group_phi(const T *vals, const T *d, const T *
 
 // (1) raw moments of f-bar+ per (face, velocity tile); one wave per pair.
 // All stencil values of an iteration are loaded before any is used.
+// b: the block's logical index in its context's grid (XCD-mapped, xcd_block).
 template <int MODEL, int D, class T, bool LIM>
-__global__ __launch_bounds__(BLOCK) void k_face_moments(KArgs a) {
+__device__ __forceinline__ void d_face_moments(const KArgs &a, const int b) {
     if (*a.bad_count > 0) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) *a.halt = 1;
+        if (b == 0 && threadIdx.x == 0) *a.halt = 1;
         return;
     }
     constexpr int WPB = BLOCK / 64;
@@ -402,7 +395,6 @@ __global__ __launch_bounds__(BLOCK) void k_face_moments(KArgs a) {
     constexpr bool SH = MODEL == CDUGKS_SHAKHOV;
     const int nf = a.s.nf;
     const int nblk = (nf + WPB - 1) / WPB;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
     const int tile = b / nblk, fb = b - tile * nblk;
     const int lane = threadIdx.x % 64;
     const int f = __builtin_amdgcn_readfirstlane(fb * WPB + int(threadIdx.x / 64));
@@ -513,6 +505,10 @@ __global__ __launch_bounds__(BLOCK) void k_face_moments(KArgs a) {
         for (int k = 0; k < NCOL; ++k) dst[k] = acc[k];
     }
 }
+template <int MODEL, int D, class T, bool LIM>
+__global__ __launch_bounds__(BLOCK) void k_face_moments(KArgs a) {
+    d_face_moments<MODEL, D, T, LIM>(a, xcd_block(blockIdx.x, gridDim.x));
+}
 
 // Axis-table entry t of equilibrium e in the storage type's form.
 template <class T>
@@ -583,8 +579,9 @@ __device__ __forceinline__ void shakhov_flux_sep(const EqParams &e, const double
 
 // (2) face macros, equilibrium constants, axis tables and flux moments of the
 // mixed value of interior faces; one wave per face
+// b: the block's logical index in its context's grid (XCD-mapped, xcd_block).
 template <int MODEL, int D, class T, bool TAB>
-__global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
+__device__ __forceinline__ void d_face_eq(const KArgs &a, const int b) {
     extern __shared__ unsigned char lds_raw[];
     if (*a.halt) return;
     constexpr int WPB = BLOCK / 64;
@@ -592,7 +589,7 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
     constexpr int NFL = D + 2;
     const int nf = a.s.nf;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-    const int f = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * WPB + wave);
+    const int f = __builtin_amdgcn_readfirstlane(b * WPB + wave);
     const bool act = f < nf;
     T *tb = reinterpret_cast<T *>(lds_raw) + size_t(wave) * a.nt;
     EqParams e;
@@ -677,7 +674,7 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
             if constexpr (sizeof(T) == 4) a.face_eqf[f] = ef;
         }
         if (TAB) {
-            T *gt = reinterpret_cast<T *>(a.face_tab) + size_t(f) * a.nt;
+            T *gt = static_cast<T *>(a.face_tab) + size_t(f) * a.nt;
             for (int t = lane; t < a.nt; t += 64) {
                 const T val = tab_entry<T>(a, e, t);
                 tb[t] = val;
@@ -768,6 +765,16 @@ __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
         }
     }
 }
+template <int MODEL, int D, class T, bool TAB>
+__global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
+    d_face_eq<MODEL, D, T, TAB>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+template <int MODEL, int D, class T, bool TAB>
+__global__ __launch_bounds__(BLOCK) void kb_face_eq(const KArgs *__restrict__ tab, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;  // padding of a short XCD list
+    d_face_eq<MODEL, D, T, TAB>(batch_args(tab, m.x), m.y);
+}
 
 // Block-wide sum of N doubles (BLOCK threads); result valid in all threads.
 template <int N>
@@ -791,11 +798,10 @@ __device__ __forceinline__ void block_sum(double (&x)[N], double *lds /* [N * BL
 
 // (3) boundary faces: one block per face
 template <int MODEL, int D, class T, bool LIM>
-__global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
+__device__ __forceinline__ void d_boundary(const KArgs &a, const int b) {
     if (*a.halt) return;
     constexpr int NFL = D + 2;
     __shared__ double lds[(NFL + 2) * (BLOCK / 64)];
-    const int b = blockIdx.x;
     const int f = a.bnd_face[b];
     const int c = a.face_cells[f].x;
     const meso_mark_desc &mk = mark_of(a, a.bnd_mark[b]);
@@ -902,6 +908,16 @@ __global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
         for (int k = 0; k < NFL; ++k) a.face_flux[size_t(f) * NFL + k] = acc[k];
         a.bnd[b] = P;
     }
+}
+template <int MODEL, int D, class T, bool LIM>
+__global__ __launch_bounds__(BLOCK) void k_boundary(KArgs a) {
+    d_boundary<MODEL, D, T, LIM>(a, blockIdx.x);
+}
+template <int MODEL, int D, class T, bool LIM>
+__global__ __launch_bounds__(BLOCK) void kb_boundary(const KArgs *__restrict__ tab, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;  // padding of a short XCD list
+    d_boundary<MODEL, D, T, LIM>(batch_args(tab, m.x), m.y);
 }
 
 // New cell macros from the face flux moments (cdugks_shakhov.cpp:644-668,
@@ -1019,7 +1035,7 @@ __device__ __forceinline__ void cell_eq_store(const KArgs &a, const EqParams &e,
 // cells that have a boundary face (launched separately, so the interior
 // kernel carries no boundary code). TAB: tensor-product DVS with axis tables.
 template <int MODEL, int D, class T, bool TAB, bool BND, bool LIM>
-__global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, int list_count) {
+__device__ __forceinline__ void d_cell_update(const KArgs &a, const int b, int list_begin, int list_count) {
     extern __shared__ unsigned char lds_raw[];
     if (*a.halt) return;
     constexpr int WPB = BLOCK / 64;
@@ -1028,7 +1044,6 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
     constexpr int GS = 1 + MAXN, NG = 1 + MAXF;
     constexpr bool SH = MODEL == CDUGKS_SHAKHOV;
     const int nblk = (list_count + WPB - 1) / WPB;
-    const int b = xcd_block(blockIdx.x, gridDim.x);
     const int tile = b / nblk, cb = b - tile * nblk;
     const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     const int li = cb * WPB + wave;
@@ -1055,7 +1070,7 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
         if (TAB) {
             const int nt = a.nt;
             for (int k = 0; k < nfc; ++k) {
-                const T *src = reinterpret_cast<const T *>(a.face_tab) + size_t(a.cs_face[c * MAXF + k]) * nt;
+                const T *src = static_cast<const T *>(a.face_tab) + size_t(a.cs_face[c * MAXF + k]) * nt;
                 for (int t = lane; t < nt; t += 64) tabs[k * nt + t] = src[t];
             }
             for (int t = lane; t < nt; t += 64) {
@@ -1337,12 +1352,16 @@ __global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, 
         for (int k = 0; k < 4; ++k) dst[k] = acc[k];
     }
 }
+template <int MODEL, int D, class T, bool TAB, bool BND, bool LIM>
+__global__ __launch_bounds__(BLOCK) void k_cell_update(KArgs a, int list_begin, int list_count) {
+    d_cell_update<MODEL, D, T, TAB, BND, LIM>(a, xcd_block(blockIdx.x, gridDim.x), list_begin, list_count);
+}
 
 // (5) q^{n+1} (Shakhov) or the DUGKS macros from the per-tile partials
 template <int MODEL>
-__global__ __launch_bounds__(BLOCK) void k_cell_finish(KArgs a) {
+__device__ __forceinline__ void d_cell_finish(const KArgs &a, const int bid) {
     if (*a.halt) return;
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    const int c = bid * BLOCK + threadIdx.x;
     if (c >= a.s.nc) return;
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     const double *src = a.part_c + size_t(c) * a.s.ntile_c * 4;
@@ -1360,6 +1379,16 @@ __global__ __launch_bounds__(BLOCK) void k_cell_finish(KArgs a) {
             atomicMin(a.bad_step, *a.step_ctr);
         }
     }
+}
+template <int MODEL>
+__global__ __launch_bounds__(BLOCK) void k_cell_finish(KArgs a) {
+    d_cell_finish<MODEL>(a, blockIdx.x);
+}
+template <int MODEL>
+__global__ __launch_bounds__(BLOCK) void kb_cell_finish(const KArgs *__restrict__ tab, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;  // padding of a short XCD list
+    d_cell_finish<MODEL>(batch_args(tab, m.x), m.y);
 }
 
 // (6) classic DUGKS: f-tilde -> f-tilde-plus with the new equilibrium (dugks.cpp:329-343)
@@ -1616,9 +1645,9 @@ __global__ __launch_bounds__(BLOCK) void k_mask_repair(KArgs a) {
                 a.T_out[c] = mac[4];
                 for (int i = 0; i < 3; ++i) {
                     a.u_out[3 * c + i] = mac[1 + i];
-                    const_cast<double *>(a.q_in)[3 * c + i] = mac[5 + i];
+                    const_cast<double *>(static_cast<const double *>(a.q_in))[3 * c + i] = mac[5 + i];
                 }
-                const_cast<int *>(a.mask)[c] = 1;
+                const_cast<int *>(static_cast<const int *>(a.mask))[c] = 1;
             }
             __syncthreads();
             EqParams e;
@@ -1642,6 +1671,11 @@ __global__ __launch_bounds__(BLOCK) void k_mask_repair(KArgs a) {
 // that record the first bad step; a captured step pair then needs no host
 // value that changes between replays).
 __global__ void k_step_end(int *step_ctr) { *step_ctr += 1; }
+// The same for every context of a batch: thread k advances context k's counter.
+__global__ void kb_step_end(int *const *__restrict__ ctrs, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) *ctrs[k] += 1;
+}
 
 #include "patch_kernels.h"
 
@@ -1672,6 +1706,10 @@ struct PatchSrc {
         return mv;
     }
 };
+
+// Device flags of a context: bad_count, first_bad, bad_step, halt, step
+// counter, masked count
+constexpr int FLAG_INTS = 6;
 
 struct meso_hip_ctx {
     StepConst s{};
@@ -1743,7 +1781,6 @@ struct meso_hip_ctx {
         *mp_rows = nullptr;
     void *crec = nullptr, *cell_tab = nullptr;
     void *fbar[2] = {nullptr, nullptr};  // patch path: face values g, h [nf][nvp] (storage type)
-    double *fbnd[2] = {nullptr, nullptr};  // FP32 patch path: boundary-face values, FP64 [nb][nvp]
     int *slot_of = nullptr;  // meso's velocity index -> storage slot (null: identity)
     int *cf_ptr = nullptr, *cf_ids = nullptr, *nb_ptr = nullptr, *nb_ids = nullptr, *face_bnd = nullptr,
         *bnd_face = nullptr, *bnd_mark = nullptr, *mirror = nullptr;
@@ -1929,8 +1966,6 @@ KArgs make_args(meso_hip_ctx *c) {
     a.xout_h = c->fx[o][1];
     a.mask = c->has_mask ? c->mask : nullptr;
     a.masked = c->flags + 5;
-    a.fbnd_g = c->fbnd[0];
-    a.fbnd_h = c->fbnd[1];
     a.fbar_g = c->fbar[0];
     a.fbar_h = c->fbar[1];
     a.rho_in = c->rho[i];
@@ -2272,7 +2307,7 @@ void set_const(StepConst &s, const meso_step_params &p) {
 }
 
 int sync_flags(meso_hip_ctx *c) {
-    HIP_TRY(c, hipMemcpyAsync(c->flags_host, c->flags, 6 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipMemcpyAsync(c->flags_host, c->flags, FLAG_INTS * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MESO_OK;
 }
@@ -2948,7 +2983,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         (rc = dalloc(c, &c->part_c, size_t(nc) * std::max(c->s.ntile_c, c->p_chunk_cap) * 4)) || (rc = dalloc(c, &c->fm, 5 * size_t(nc))) ||
         (rc = dalloc(c, &c->res_part, 8 * size_t((nc + BLOCK - 1) / BLOCK))) || (rc = dalloc(c, &c->res_out, 8)) ||
         (rc = dalloc(c, &c->face_eq, nf)) || (rc = dalloc(c, &c->face_eqf, nf)) || (rc = dalloc(c, &c->bnd, std::max(1, c->s.nb))) ||
-        (rc = dalloc(c, &c->flags, 6)) || (rc = dalloc(c, &c->mask, nc)) || (rc = dalloc(c, &c->fref_d, 2 * size_t(nv))) ||
+        (rc = dalloc(c, &c->flags, FLAG_INTS)) || (rc = dalloc(c, &c->mask, nc)) || (rc = dalloc(c, &c->fref_d, 2 * size_t(nv))) ||
         (rc = dalloc(c, &c->fref_f, 2 * size_t(nv))) || (rc = dalloc(c, &c->fref_d2, nv)) ||
         (rc = dalloc(c, &c->fref_f2, nv)))
         return fail(rc);
@@ -2973,9 +3008,6 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             char *fb = nullptr;
             if ((rc = dalloc(c, &fb, size_t(nf) * c->s.nvp * es))) return fail(rc);
             c->fbar[k] = fb;
-            if (storage == MESO_STORAGE_FP32 && c->s.nb > 0 &&
-                (rc = dalloc(c, &c->fbnd[k], size_t(c->s.nb) * c->s.nvp)))
-                return fail(rc);
         }
     }
     c->xi_h = xi_s;
@@ -2984,7 +3016,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         return fail(set_err(nullptr, MESO_E_INVALID,
                             "MESO_STORAGE_FP32X needs the patch kernels: a CDUGKS model, at least 64 velocities "
                             "and no symmetry or slip-wall boundary"));
-    if (hipHostMalloc((void **)&c->flags_host, 6 * sizeof(int), hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void **)&c->flags_host, FLAG_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(set_err(nullptr, MESO_E_DEVICE, "hipHostMalloc failed"));
     if ((rc = reset_flags(c))) return fail(rc);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_err(nullptr, MESO_E_DEVICE, "upload failed"));
@@ -3027,8 +3059,11 @@ int meso_hip_set_mask(meso_hip_ctx *c, const int32_t *mask) {
 int meso_hip_set_params(meso_hip_ctx *c, const meso_step_params *p) {
     if (!c || !p) return set_err(c, MESO_E_INVALID, "null argument");
     if (p->model != c->s.model || p->dim != c->s.D) return set_err(c, MESO_E_INVALID, "model/dimension cannot change");
+    const StepConst before = c->s;
     set_const(c->s, *p);
-    ++c->kver;
+    // meso re-reads its settings every step (cdugks_shakhov.cpp:762): unchanged
+    // values keep the captured steps and batch plans
+    if (std::memcmp(&before, &c->s, sizeof(StepConst)) != 0) ++c->kver;
     return MESO_OK;
 }
 
@@ -3236,6 +3271,7 @@ int meso_hip_get_state(meso_hip_ctx *c, double *f, int32_t layout) {
 }
 
 // Launches one step (no synchronisation).
+
 static int step_one(meso_hip_ctx *c) {
     KArgs a = make_args(c);
     const int rc = step_dispatch(c, a);
@@ -3324,6 +3360,19 @@ int meso_hip_step(meso_hip_ctx *c, int32_t nsteps, meso_step_report *rep) {
 int meso_hip_set_graphs(meso_hip_ctx *c, int32_t enable) {
     if (!c) return set_err(c, MESO_E_INVALID, "null context");
     c->graphs = enable != 0;
+    return MESO_OK;
+}
+
+int meso_hip_set_vchunk(meso_hip_ctx *c, int32_t vchunk) {
+    if (!c) return set_err(c, MESO_E_INVALID, "null context");
+    if (!c->patches) return vchunk == 0 ? MESO_OK : set_err(c, MESO_E_INVALID, "the case runs the per-cell kernels");
+    const int nv = c->s.nv;
+    int ncu = 256;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device);
+    const int vc = vchunk == 0 ? default_vchunk(nv, std::min(c->n_up, c->n_mp), ncu) : vchunk;
+    if (vc < 64 || vc % 64 != 0) return set_err(c, MESO_E_INVALID, "vchunk must be a positive multiple of 64");
+    if ((nv + vc - 1) / vc > c->p_chunk_cap) return set_err(c, MESO_E_INVALID, "vchunk too small for the partial-sum buffers");
+    if (vc != c->p_vchunk) set_vchunk(c, vc);
     return MESO_OK;
 }
 
@@ -3461,6 +3510,8 @@ int meso_hip_tiles(const meso_hip_ctx *c, meso_tile_config *out) {
         t.moment_by_cells = c->mp_cells ? 1 : 0;
         t.moment_size = c->mp_size;
         t.vchunk = c->p_vchunk;
+        t.moment_lds = int(c->lds_mp);
+        t.update_lds = int(c->lds_up);
         const bool f32 = c->storage == MESO_STORAGE_FP32;
         t.moment_vt = f32 ? 64 : 32;
         t.update_vt = c->s.D == 2 ? (f32 ? upd_vt<float, 2>() : upd_vt<double, 2>())
@@ -3570,6 +3621,579 @@ int meso_hip_sync(meso_hip_ctx *c) {
     if (!c) return set_err(c, MESO_E_INVALID, "null context");
     HIP_TRY(c, hipStreamSynchronize(c->stream));
     return MESO_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== fused batch
+// A fixed set of contexts advanced together: the contexts whose step runs the
+// patch kernels are grouped by kernel instance, and each stage of a step is
+// one launch over the blocks of every context of a group. A block reads its
+// (context, block) pair from a per-block map and the context's kernel
+// arguments from a device table, so every context computes exactly what its
+// own launches compute (same blocks, same arithmetic: bit-identical).
+namespace {
+
+enum BatchStage { BS_MOM, BS_FEQ, BS_BND, BS_PREP, BS_UPD, BS_FIN, BS_N };
+
+struct BatchGroup {
+    int sub = 0;                         // the sub-batch (stream) that launches it
+    std::vector<int> members;            // positions in meso_hip_batch::ctx
+    int blocks[BS_N] = {};               // grid of each stage (0: no launch)
+    int2 *map[BS_N] = {};                // per block: (member, block of its own grid)
+    KArgs *tab[2] = {nullptr, nullptr};  // per phase: the members' arguments
+    size_t lds_mp = 0, lds_up = 0, lds_feq = 0;
+};
+
+// Key of the kernel instance a context's step launches (contexts that share
+// it share the launches).
+long batch_key(const meso_hip_ctx *c) {
+    return ((((long(c->s.model) * 4 + c->s.D) * 2 + (c->storage == MESO_STORAGE_FP32)) * 2 + c->ext) * 2 +
+            (c->nt > 0)) * 4 + c->s.limiter * 2 + c->mp_cells;
+}
+
+// A context can join the fused launches when its whole step is the patch
+// path: no mirror-boundary cells (their own per-cell launch), no masked
+// groups (the repair launch), not profiling (per-launch events).
+bool batch_fusable(const meso_hip_ctx *c) {
+    return c->patches && c->s.model != DUGKS_BGK && c->n_bcell == 0 && c->n_int > 0 && !c->has_mask &&
+           !c->profiling;
+}
+
+// Per-block map of one stage over the members: member k has n[k] blocks of
+// work w[k] each, in its own logical block order. Hardware block B runs on XCD
+// B % 8, so the j-th block of XCD x is B = 8 j + x. Every XCD takes a
+// contiguous eighth of every member's blocks (neighbouring patches share that
+// XCD's L2, and each XCD gets the same mix of light and heavy blocks whatever
+// their real cost), members with the heaviest blocks first. Short XCD lists
+// are padded with (-1, -1).
+std::vector<int2> batch_map(const std::vector<int> &n, const std::vector<double> &w) {
+    constexpr int NX = 8;
+    const int m = int(n.size());
+    std::vector<int> order(m);
+    for (int k = 0; k < m; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return w[x] > w[y]; });
+    std::vector<std::vector<int2>> lists(NX);
+    for (int k : order)
+        for (int x = 0; x < NX; ++x) {
+            const int b0 = int(int64_t(n[k]) * x / NX), b1 = int(int64_t(n[k]) * (x + 1) / NX);
+            for (int b = b0; b < b1; ++b) lists[x].push_back(make_int2(k, b));
+        }
+    size_t len = 0;
+    for (auto &l : lists) len = std::max(len, l.size());
+    std::vector<int2> map(len * NX, make_int2(-1, -1));
+    for (int x = 0; x < NX; ++x)
+        for (size_t j = 0; j < lists[x].size(); ++j) map[j * NX + x] = lists[x][j];
+    return map;
+}
+
+}  // namespace
+
+struct meso_hip_batch {
+    std::vector<meso_hip_ctx *> ctx;
+    std::vector<int> solo;               // contexts stepped with their own launches
+    std::vector<BatchGroup> groups;
+    std::vector<int> kver0, cur0;        // per context, at the last build
+    // sub-batches: the fused contexts are split (by DV-cells) over up to
+    // nsub_req streams; each sub-batch launches its groups' stages on its own
+    // stream, so the small stages of one overlap the large ones of another
+    int nsub_req = 4;
+    int nsub = 0;
+    std::vector<int **> ctrs;            // per sub-batch: its contexts' step counters
+    std::vector<int> nctr;
+    int nfused = 0;
+    int phase = 0;                       // fused steps since the build, mod 2
+    bool built = false;
+    int device = 0;
+    hipStream_t stream = nullptr;        // origin stream (sub-batch 0)
+    hipEvent_t ev = nullptr;
+    std::vector<hipStream_t> sstream;    // sub-batch streams 1.. (0 is `stream`)
+    std::vector<hipEvent_t> sev;         // join events of the sub-batch streams
+    bool graphs = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    std::vector<void *> dmem;            // device memory of the plan
+    bool rechunk = true;                 // set the fused contexts' velocity chunks for the batch grid
+    std::string err;
+};
+
+namespace {
+
+int berr(meso_hip_batch *b, int code, const std::string &m) {
+    if (b) b->err = m;
+    g_hip_err = m;
+    return code;
+}
+
+#define BHIP(b, expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess) return berr(b, MESO_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void batch_release(meso_hip_batch *b) {
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (hipStream_t st : b->sstream) (void)hipStreamSynchronize(st);
+    for (auto &g : b->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    for (void *p : b->dmem) (void)hipFree(p);
+    b->dmem.clear();
+    b->groups.clear();
+    b->solo.clear();
+    b->ctrs.clear();
+    b->nctr.clear();
+    b->nfused = 0;
+    b->nsub = 0;
+    b->built = false;
+}
+
+template <class P>
+int batch_upload(meso_hip_batch *b, P **dst, const std::vector<P> &v) {
+    void *q = nullptr;
+    BHIP(b, hipMalloc(&q, std::max<size_t>(v.size() * sizeof(P), 16)));
+    b->dmem.push_back(q);
+    if (!v.empty()) BHIP(b, hipMemcpyAsync(q, v.data(), v.size() * sizeof(P), hipMemcpyHostToDevice, b->stream));
+    *dst = reinterpret_cast<P *>(q);
+    return MESO_OK;
+}
+
+// Velocity chunk of a context in the fused grid. Alone, a context takes
+// enough chunks that its patch grid has 16 blocks per CU (default_vchunk);
+// in a batch the contexts fill the grid together, so each takes only the
+// chunks its share of the batch's DV-cells needs (at least one per 2560
+// velocities): fewer, larger blocks, and fewer partial sums per face for
+// k_face_eq to add (the 64-case sweep: up to 128 partial sums per face
+// otherwise, and its face_eq launch 10 ms).
+int batch_vchunk(const meso_hip_ctx *c, double share, int ncu) {
+    const int nv = c->s.nv;
+    const int npatch = std::max(1, std::min(c->n_up, c->n_mp));
+    int chunks = std::max((nv + 2559) / 2560, int(std::ceil(share * 16.0 * ncu / npatch)));
+    chunks = std::max(1, std::min({chunks, (nv + 63) / 64, c->p_chunk_cap}));
+    return ((nv + chunks - 1) / chunks + 63) / 64 * 64;
+}
+
+// Groups, per-stage block maps and argument tables from the contexts' current
+// state; phase 0 is their current buffer parity. The fused contexts' velocity
+// chunks are set for the batch grid first (batch_vchunk).
+int batch_build(meso_hip_batch *b) {
+    batch_release(b);
+    constexpr int WPB = BLOCK / 64;
+    const int n = int(b->ctx.size());
+    {
+        int ncu = 256;
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device);
+        double total = 0.0;
+        for (meso_hip_ctx *c : b->ctx)
+            if (batch_fusable(c)) total += double(c->s.nc) * c->s.nv;
+        for (meso_hip_ctx *c : b->ctx)
+            if (batch_fusable(c) && b->rechunk) {
+                const int vc = batch_vchunk(c, double(c->s.nc) * c->s.nv / total, ncu);
+                if (vc != c->p_vchunk) set_vchunk(c, vc);
+            }
+    }
+    b->kver0.assign(n, 0);
+    b->cur0.assign(n, 0);
+    std::vector<int> fus;
+    for (int k = 0; k < n; ++k) {
+        meso_hip_ctx *c = b->ctx[k];
+        b->kver0[k] = c->kver;
+        b->cur0[k] = c->cur;
+        if (batch_fusable(c)) fus.push_back(k);
+        else b->solo.push_back(k);
+    }
+    b->nfused = int(fus.size());
+    b->nsub = std::max(1, std::min(b->nsub_req, b->nfused));
+    // sub-batches: largest context first to the least loaded
+    std::vector<int> sub_of(n, 0);
+    {
+        std::vector<int> ord = fus;
+        auto dv = [&](int k) { return double(b->ctx[k]->s.nc) * b->ctx[k]->s.nv; };
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return dv(x) > dv(y); });
+        std::vector<double> load(b->nsub, 0.0);
+        for (int k : ord) {
+            const int sb = int(std::min_element(load.begin(), load.end()) - load.begin());
+            sub_of[k] = sb;
+            load[sb] += dv(k);
+        }
+    }
+    std::vector<std::pair<int, long>> keys;
+    std::vector<std::vector<int *>> ctrs(b->nsub);
+    for (int k : fus) {
+        meso_hip_ctx *c = b->ctx[k];
+        const std::pair<int, long> key(sub_of[k], batch_key(c));
+        size_t g = 0;
+        while (g < keys.size() && keys[g] != key) ++g;
+        if (g == keys.size()) {
+            keys.push_back(key);
+            b->groups.emplace_back();
+            b->groups.back().sub = key.first;
+        }
+        b->groups[g].members.push_back(k);
+        ctrs[sub_of[k]].push_back(c->flags + 4);
+    }
+    b->ctrs.assign(b->nsub, nullptr);
+    b->nctr.assign(b->nsub, 0);
+    for (int sb = 0; sb < b->nsub; ++sb) {
+        b->nctr[sb] = int(ctrs[sb].size());
+        if (b->nctr[sb]) {
+            int rc = batch_upload(b, &b->ctrs[sb], ctrs[sb]);
+            if (rc) return rc;
+        }
+    }
+    while (int(b->sstream.size()) < b->nsub - 1) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        BHIP(b, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        b->sstream.push_back(st);
+        BHIP(b, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        b->sev.push_back(ev);
+    }
+    for (BatchGroup &g : b->groups) {
+        const int m = int(g.members.size());
+        std::vector<int> nb[BS_N];
+        std::vector<double> wb[BS_N];
+        std::vector<KArgs> tab[2];
+        for (int i = 0; i < m; ++i) {
+            meso_hip_ctx *c = b->ctx[g.members[i]];
+            const StepConst &s = c->s;
+            const double dvc = double(s.nc) * s.nv;
+            const int cnt[BS_N] = {c->n_mp * c->p_chunks, (s.nf + WPB - 1) / WPB, s.nb, (c->n_int + WPB - 1) / WPB,
+                                   c->n_up * c->p_chunks,
+                                   s.model == CDUGKS_SHAKHOV ? (s.nc + BLOCK - 1) / BLOCK : 0};
+            const double work[BS_N] = {dvc, double(s.nf) * std::max(c->nt, 1), double(s.nb) * s.nv,
+                                       double(c->n_int) * std::max(c->nt, 1), dvc, double(s.nc)};
+            for (int st = 0; st < BS_N; ++st) {
+                nb[st].push_back(cnt[st]);
+                wb[st].push_back(cnt[st] > 0 ? work[st] / cnt[st] : 0.0);
+            }
+            g.lds_mp = std::max(g.lds_mp, c->lds_mp);
+            g.lds_up = std::max(g.lds_up, c->lds_up);
+            const size_t tsz = c->storage == MESO_STORAGE_FP32 ? 4 : 8;
+            g.lds_feq = std::max(g.lds_feq, c->nt > 0 ? size_t(WPB) * c->nt * tsz : size_t(0));
+            for (int p = 0; p < 2; ++p) {
+                const int cur = c->cur;
+                c->cur = cur ^ p;
+                tab[p].push_back(make_args(c));
+                c->cur = cur;
+            }
+        }
+        for (int st = 0; st < BS_N; ++st) {
+            const std::vector<int2> map = batch_map(nb[st], wb[st]);
+            bool any = false;
+            for (int x : nb[st]) any = any || x > 0;
+            g.blocks[st] = any ? int(map.size()) : 0;
+            if (!any) continue;
+            int rc = batch_upload(b, &g.map[st], map);
+            if (rc) return rc;
+        }
+        for (int p = 0; p < 2; ++p) {
+            int rc = batch_upload(b, &g.tab[p], tab[p]);
+            if (rc) return rc;
+        }
+    }
+    BHIP(b, hipStreamSynchronize(b->stream));
+    b->phase = 0;
+    b->built = true;
+    return MESO_OK;
+}
+
+// The plan is stale when a context's kernel arguments changed, or it was
+// stepped on its own, since the build.
+bool batch_stale(const meso_hip_batch *b) {
+    if (!b->built) return true;
+    for (size_t k = 0; k < b->ctx.size(); ++k) {
+        const meso_hip_ctx *c = b->ctx[k];
+        if (c->kver != b->kver0[k] || c->cur != (b->cur0[k] ^ b->phase)) return true;
+        if (batch_fusable(c) != (std::find(b->solo.begin(), b->solo.end(), int(k)) == b->solo.end())) return true;
+    }
+    return false;
+}
+
+#define BLAUNCH(st, kern, grid, block, shmem, ...) \
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), shmem, st, __VA_ARGS__)
+
+hipStream_t sub_stream(const meso_hip_batch *b, int sb) { return sb == 0 ? b->stream : b->sstream[sb - 1]; }
+
+// The sub-batch streams follow the origin stream (fork) ...
+int batch_fork(meso_hip_batch *b) {
+    if (b->nsub < 2) return MESO_OK;
+    BHIP(b, hipEventRecord(b->ev, b->stream));
+    for (int sb = 1; sb < b->nsub; ++sb) BHIP(b, hipStreamWaitEvent(sub_stream(b, sb), b->ev, 0));
+    return MESO_OK;
+}
+
+// ... and the origin stream follows them (join).
+int batch_join(meso_hip_batch *b) {
+    for (int sb = 1; sb < b->nsub; ++sb) {
+        BHIP(b, hipEventRecord(b->sev[sb - 1], sub_stream(b, sb)));
+        BHIP(b, hipStreamWaitEvent(b->stream, b->sev[sb - 1], 0));
+    }
+    return MESO_OK;
+}
+
+template <class K>
+int batch_lds_attr(meso_hip_batch *b, K *kern, size_t lds) {
+    if (lds > 65536)
+        BHIP(b, hipFuncSetAttribute(reinterpret_cast<const void *>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lds)));
+    return MESO_OK;
+}
+
+// One step of a group: the launches of step_patch, each over every member.
+template <int MODEL, class T, bool TAB, int D, bool LIM, bool X>
+int batch_group_step(meso_hip_batch *b, const BatchGroup &g, int phase, bool cells) {
+    constexpr int VTM = sizeof(T) == 4 ? 64 : 32;
+    constexpr int VTU = upd_vt<T, D>();
+    const KArgs *tab = g.tab[phase];
+    const hipStream_t st = sub_stream(b, g.sub);
+    int rc;
+    if (cells) {
+        auto *km = &kb_patch_moments<MODEL, T, VTM, D, LIM, true, X>;
+        if ((rc = batch_lds_attr(b, km, g.lds_mp))) return rc;
+        BLAUNCH(st, km, g.blocks[BS_MOM], BLOCKP, g.lds_mp, tab, g.map[BS_MOM]);
+    } else {
+        auto *km = &kb_patch_moments<MODEL, T, VTM, D, LIM, false, X>;
+        if ((rc = batch_lds_attr(b, km, g.lds_mp))) return rc;
+        BLAUNCH(st, km, g.blocks[BS_MOM], BLOCKP, g.lds_mp, tab, g.map[BS_MOM]);
+    }
+    BLAUNCH(st, (kb_face_eq<MODEL, D, T, TAB>), g.blocks[BS_FEQ], BLOCK, TAB ? g.lds_feq : 0, tab, g.map[BS_FEQ]);
+    if (g.blocks[BS_BND]) BLAUNCH(st, (kb_boundary<MODEL, D, T, LIM>), g.blocks[BS_BND], BLOCK, 0, tab, g.map[BS_BND]);
+    BLAUNCH(st, (kb_cell_prep<MODEL, T, TAB, D>), g.blocks[BS_PREP], BLOCK, 0, tab, g.map[BS_PREP]);
+    auto *ku = &kb_patch_update<MODEL, T, VTU, TAB, D, LIM, X>;
+    if ((rc = batch_lds_attr(b, ku, g.lds_up))) return rc;
+    BLAUNCH(st, ku, g.blocks[BS_UPD], BLOCKP, g.lds_up, tab, g.map[BS_UPD]);
+    if (MODEL == CDUGKS_SHAKHOV) BLAUNCH(st, (kb_cell_finish<MODEL>), g.blocks[BS_FIN], BLOCK, 0, tab, g.map[BS_FIN]);
+    BHIP(b, hipGetLastError());
+    return MESO_OK;
+}
+
+int batch_group_dispatch(meso_hip_batch *b, const BatchGroup &g, int phase) {
+    const meso_hip_ctx *c = b->ctx[g.members[0]];
+    const int m = c->s.model, D = c->s.D;
+    // as step_dispatch: axis tables and the limiter only for Shakhov
+    const bool sh = m == CDUGKS_SHAKHOV;
+    const bool f32 = c->storage == MESO_STORAGE_FP32, tab = sh && c->nt > 0, lim = sh && c->s.limiter != 0,
+               x = c->ext, cells = c->mp_cells;
+#define MESO_BL(M, DD, TB, L)                                                                                    \
+    if (m == M && D == DD && tab == TB && lim == L) {                                                            \
+        if (f32) return batch_group_step<M, float, TB, DD, L, false>(b, g, phase, cells);                       \
+        return x ? batch_group_step<M, double, TB, DD, L, true>(b, g, phase, cells)                             \
+                 : batch_group_step<M, double, TB, DD, L, false>(b, g, phase, cells);                           \
+    }
+#define MESO_BD(DD)                                                                                              \
+    MESO_BL(CDUGKS_SHAKHOV, DD, true, false) MESO_BL(CDUGKS_SHAKHOV, DD, true, true)                             \
+    MESO_BL(CDUGKS_SHAKHOV, DD, false, false) MESO_BL(CDUGKS_SHAKHOV, DD, false, true)                           \
+    MESO_BL(CDUGKS_BGK, DD, false, false)
+    MESO_BD(2)
+    MESO_BD(3)
+#undef MESO_BD
+#undef MESO_BL
+    return berr(b, MESO_E_INVALID, "batch: unsupported kernel instance");
+}
+
+// One fused step of every group (and the step counters), then the host-side
+// buffer parity of the fused contexts.
+int batch_fused_step(meso_hip_batch *b) {
+    for (const BatchGroup &g : b->groups) {
+        const int rc = batch_group_dispatch(b, g, b->phase);
+        if (rc) return rc;
+    }
+    for (int sb = 0; sb < b->nsub; ++sb)
+        if (b->nctr[sb]) BLAUNCH(sub_stream(b, sb), kb_step_end, (b->nctr[sb] + 63) / 64, 64, 0, b->ctrs[sb], b->nctr[sb]);
+    BHIP(b, hipGetLastError());
+    for (const BatchGroup &g : b->groups)
+        for (int k : g.members) b->ctx[k]->cur ^= 1;
+    b->phase ^= 1;
+    return MESO_OK;
+}
+
+// Two fused steps from the current phase as a graph (captured on first use).
+int batch_pair_graph(meso_hip_batch *b) {
+    const int p = b->phase;
+    if (b->gexec[p]) return MESO_OK;
+    std::vector<int> cur(b->ctx.size());
+    for (size_t k = 0; k < b->ctx.size(); ++k) cur[k] = b->ctx[k]->cur;
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_fork(b);
+    if (rc == MESO_OK) rc = batch_fused_step(b);
+    if (rc == MESO_OK) rc = batch_fused_step(b);
+    if (rc == MESO_OK) rc = batch_join(b);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    // capture launched nothing: undo the host-side parities of the captured steps
+    b->phase = p;
+    for (size_t k = 0; k < b->ctx.size(); ++k) b->ctx[k]->cur = cur[k];
+    if (rc == MESO_OK && e != hipSuccess)
+        rc = berr(b, MESO_E_DEVICE, std::string("batch capture: ") + hipGetErrorString(e));
+    if (rc == MESO_OK) {
+        const hipError_t ei = hipGraphInstantiate(&b->gexec[p], gr, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+            b->gexec[p] = nullptr;
+            rc = berr(b, MESO_E_DEVICE, std::string("batch graph instantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    if (gr) (void)hipGraphDestroy(gr);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int meso_hip_batch_create(meso_hip_ctx *const *ctxs, int32_t n, meso_hip_batch **out) {
+    if (!ctxs || n <= 0 || !out) return set_err(nullptr, MESO_E_INVALID, "bad argument");
+    *out = nullptr;
+    for (int k = 0; k < n; ++k) {
+        if (!ctxs[k]) return set_err(nullptr, MESO_E_INVALID, "null context in the batch");
+        if (ctxs[k]->device != ctxs[0]->device)
+            return set_err(nullptr, MESO_E_INVALID, "batch contexts must share one device");
+        for (int j = 0; j < k; ++j)
+            if (ctxs[j] == ctxs[k]) return set_err(nullptr, MESO_E_INVALID, "a context appears twice in the batch");
+    }
+    auto *b = new meso_hip_batch();
+    b->ctx.assign(ctxs, ctxs + n);
+    b->device = ctxs[0]->device;
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        const int rc = set_err(nullptr, MESO_E_DEVICE, std::string("batch stream: ") + hipGetErrorString(e));
+        meso_hip_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return MESO_OK;
+}
+
+void meso_hip_batch_destroy(meso_hip_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    batch_release(b);
+    for (hipEvent_t e : b->sev) (void)hipEventDestroy(e);
+    for (hipStream_t st : b->sstream) (void)hipStreamDestroy(st);
+    if (b->ev) (void)hipEventDestroy(b->ev);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int meso_hip_batch_set_graphs(meso_hip_batch *b, int32_t enable) {
+    if (!b) return set_err(nullptr, MESO_E_INVALID, "null batch");
+    b->graphs = enable != 0;
+    return MESO_OK;
+}
+
+int meso_hip_batch_set_streams(meso_hip_batch *b, int32_t n) {
+    if (!b || n < 1 || n > 16) return set_err(nullptr, MESO_E_INVALID, "bad argument");
+    if (n != b->nsub_req) b->built = false;
+    b->nsub_req = n;
+    return MESO_OK;
+}
+
+int meso_hip_batch_set_rechunk(meso_hip_batch *b, int32_t enable) {
+    if (!b) return set_err(nullptr, MESO_E_INVALID, "null batch");
+    if ((enable != 0) != b->rechunk) b->built = false;
+    b->rechunk = enable != 0;
+    return MESO_OK;
+}
+
+int meso_hip_batch_info(meso_hip_batch *b, int32_t *fused, int32_t *groups, int32_t *launches_per_step) {
+    if (!b) return set_err(nullptr, MESO_E_INVALID, "null batch");
+    if (batch_stale(b)) {
+        (void)hipSetDevice(b->device);
+        const int rc = batch_build(b);
+        if (rc) return rc;
+    }
+    int l = 0;
+    for (int sb = 0; sb < b->nsub; ++sb) l += b->nctr[sb] > 0;
+    for (const BatchGroup &g : b->groups)
+        for (int st = 0; st < BS_N; ++st) l += g.blocks[st] > 0;
+    if (fused) *fused = b->nfused;
+    if (groups) *groups = int(b->groups.size());
+    if (launches_per_step) *launches_per_step = l;
+    return MESO_OK;
+}
+
+int meso_hip_batch_step(meso_hip_batch *b, int32_t nsteps, meso_step_report *reports) {
+    if (!b || nsteps < 0) return set_err(nullptr, MESO_E_INVALID, "bad argument");
+    BHIP(b, hipSetDevice(b->device));
+    if (batch_stale(b)) {
+        const int rc = batch_build(b);
+        if (rc) return rc;
+    }
+    const int n = int(b->ctx.size());
+    // the batch stream follows earlier work on the contexts' streams
+    for (int k = 0; k < n; ++k) {
+        BHIP(b, hipEventRecord(b->ev, b->ctx[k]->stream));
+        BHIP(b, hipStreamWaitEvent(b->stream, b->ev, 0));
+    }
+    // the sub-batch streams follow the origin for eager steps (forked); a
+    // graph replay runs on the origin stream and forks inside, so the origin
+    // first takes in the sub-batch streams' eager work (joined)
+    bool forked = false;
+    std::vector<int> rc(n, MESO_OK), issued(n, 0);
+    int frc = MESO_OK, fdone = 0;
+    // fused steps on the batch streams, round-robin with the solo contexts' steps
+    for (bool more = true; more;) {
+        more = false;
+        if (frc == MESO_OK && fdone < nsteps && b->nfused) {
+            const bool graph = b->graphs && nsteps - fdone >= 2;
+            if (graph == forked) {
+                frc = graph ? batch_join(b) : batch_fork(b);
+                forked = !graph;
+            }
+            if (frc != MESO_OK) {
+            } else if (graph) {
+                frc = batch_pair_graph(b);
+                if (frc == MESO_OK) {
+                    const hipError_t e = hipGraphLaunch(b->gexec[b->phase], b->stream);
+                    if (e != hipSuccess) frc = berr(b, MESO_E_DEVICE, std::string("batch graph launch: ") + hipGetErrorString(e));
+                }
+                if (frc == MESO_OK) fdone += 2;  // two steps: parities and phase are back where they were
+            } else {
+                frc = batch_fused_step(b);
+                if (frc == MESO_OK) ++fdone;
+            }
+            more = more || (frc == MESO_OK && fdone < nsteps);
+        }
+        for (int k : b->solo)
+            if (rc[k] == MESO_OK && issued[k] < nsteps) {
+                int got = 0;
+                rc[k] = step_issue(b->ctx[k], nsteps - issued[k], &got);
+                issued[k] += got;
+                more = more || (rc[k] == MESO_OK && issued[k] < nsteps);
+            }
+    }
+    std::vector<char> fused(n, 0);
+    for (const BatchGroup &g : b->groups)
+        for (int k : g.members) {
+            issued[k] = fdone;
+            rc[k] = frc;
+            fused[k] = 1;
+        }
+    // the contexts' streams (their reports, later calls) follow the batch
+    if (forked) {
+        const int rj = batch_join(b);
+        if (rj) return rj;
+    }
+    BHIP(b, hipEventRecord(b->ev, b->stream));
+    for (int k = 0; k < n; ++k) BHIP(b, hipStreamWaitEvent(b->ctx[k]->stream, b->ev, 0));
+    int first = MESO_OK;
+    for (int k = 0; k < n; ++k) {
+        meso_step_report r{};
+        int f = step_finish(b->ctx[k], issued[k], &r);
+        if (rc[k] != MESO_OK) f = rc[k];
+        if (reports) reports[k] = r;
+        if (first == MESO_OK && f != MESO_OK) {
+            first = f;
+            const std::string msg = rc[k] != MESO_OK && fused[k] ? b->err : b->ctx[k]->err;
+            berr(b, f, msg);
+        }
+    }
+    // a halted context's buffers: step_finish put its parity back, and the
+    // plan follows it at the next call (stale)
+    return first;
 }
 
 }  // extern "C"

@@ -28,125 +28,149 @@
 
 namespace meso_dev {
 
-// Everything a kernel needs; passed by value.
+// A pointer into device memory as a kernel argument member: it holds the
+// global-address-space pointer and converts to an ordinary pointer where it
+// is used, so accesses through it compile to global instructions whether the
+// arguments come by value or from a table in memory (where a plain pointer
+// member would give flat instructions, which also wait on the LDS counter).
+template <class T>
+struct gptr {
+    __attribute__((address_space(1))) T *p;
+    __host__ __device__ operator T *() const { return (T *)p; }
+    template <class U>
+    __host__ __device__ explicit operator U *() const {
+        return (U *)(T *)p;
+    }
+    template <class U>
+    __host__ __device__ U *as() const {
+        return (U *)(T *)p;
+    }
+    __host__ __device__ gptr &operator=(T *q) {
+        p = (__attribute__((address_space(1))) T *)q;
+        return *this;
+    }
+};
+
+// Everything a kernel needs; passed by value, or read from a device table by
+// the fused batch kernels.
 struct KArgs {
     StepConst s;
     // mesh
-    const int2 *face_cells;     // [nf] owner, neighbour
-    const double *face_n;       // [nf*3] unit normal out of the owner
-    const double *face_d;       // [nf*6] x_f - x_owner, x_f - x_neighbour
-    const float *face_df;       // float copy of face_d
-    const int *cf_ptr;          // [nc+1] cell -> faces
-    const int *cf_ids;          // faces
-    const double *cf_sA;        // +-area (+ when the cell owns the face)
-    const int *nb_ptr;          // [nc+1] LSQ neighbours
-    const int *nb_ids;
-    const double *nb_a;         // [nnz*3] w_j (Cx.dr_j, Cy.dr_j, Cz.dr_j)
-    const double *nb_w;         // [nnz] w_j = 1/|dr_j|^2 (inverse-distance weights of the mask repair)
-    const float *nb_af;
-    const double *cell_vol;
-    const double *cell_S;       // [nc*3] sum over the cell's faces of +-A n (0 for a closed, planar-faced cell)
-    const int *face_bnd;        // [nf] boundary index or -1
+    gptr<const int2> face_cells; // [nf] owner, neighbour
+    gptr<const double> face_n;  // [nf*3] unit normal out of the owner
+    gptr<const double> face_d;  // [nf*6] x_f - x_owner, x_f - x_neighbour
+    gptr<const float> face_df;  // float copy of face_d
+    gptr<const int> cf_ptr;     // [nc+1] cell -> faces
+    gptr<const int> cf_ids;     // faces
+    gptr<const double> cf_sA;   // +-area (+ when the cell owns the face)
+    gptr<const int> nb_ptr;     // [nc+1] LSQ neighbours
+    gptr<const int> nb_ids;
+    gptr<const double> nb_a;    // [nnz*3] w_j (Cx.dr_j, Cy.dr_j, Cz.dr_j)
+    gptr<const double> nb_w;    // [nnz] w_j = 1/|dr_j|^2 (inverse-distance weights of the mask repair)
+    gptr<const float> nb_af;
+    gptr<const double> cell_vol;
+    gptr<const double> cell_S;  // [nc*3] sum over the cell's faces of +-A n (0 for a closed, planar-faced cell)
+    gptr<const int> face_bnd;   // [nf] boundary index or -1
     // per-cell stencil tables, padded to MAXF faces / MAXN neighbours
     // (2D: 4, 3D: 6), in the cell's local face order
-    const int *cs_nf;           // [nc] faces of the cell
-    const int *cs_face;         // [nc*MAXF] face id
-    const int *cs_other;        // [nc*MAXF] cell across the face, -1 on a boundary
-    const int *cs_own;          // [nc*MAXF] 1 when the cell owns the face
-    const int *cs_bnd;          // [nc*MAXF] boundary index or -1
-    const double *cs_sA;        // [nc*MAXF] +-area
-    const double *cs_d;         // [nc*MAXF*6] x_f - x_c, x_f - x_other
-    const float *cs_df;
-    const int *lsq_cnt;         // [nc] LSQ neighbours
-    const int *lsq_nb;          // [nc*MAXN]
-    const double *lsq_a;        // [nc*MAXN*3] w_j (Cx.dr_j, Cy.dr_j, Cz.dr_j)
-    const float *lsq_af;
-    const float *face_nf;       // [nf*3] float copy of the normals
-    EqF *face_eqf;              // [nf] FP32 copy of face_eq
+    gptr<const int> cs_nf;      // [nc] faces of the cell
+    gptr<const int> cs_face;    // [nc*MAXF] face id
+    gptr<const int> cs_other;   // [nc*MAXF] cell across the face, -1 on a boundary
+    gptr<const int> cs_own;     // [nc*MAXF] 1 when the cell owns the face
+    gptr<const int> cs_bnd;     // [nc*MAXF] boundary index or -1
+    gptr<const double> cs_sA;   // [nc*MAXF] +-area
+    gptr<const double> cs_d;    // [nc*MAXF*6] x_f - x_c, x_f - x_other
+    gptr<const float> cs_df;
+    gptr<const int> lsq_cnt;    // [nc] LSQ neighbours
+    gptr<const int> lsq_nb;     // [nc*MAXN]
+    gptr<const double> lsq_a;   // [nc*MAXN*3] w_j (Cx.dr_j, Cy.dr_j, Cz.dr_j)
+    gptr<const float> lsq_af;
+    gptr<const float> face_nf;  // [nf*3] float copy of the normals
+    gptr<EqF> face_eqf;         // [nf] FP32 copy of face_eq
     // velocity slots
-    const double4 *vel_d;       // [nv] x, y, z, w
-    const float4 *vel_f;        // [nv] x, y, z, |xi|^2
-    const double2 *fref_d2;     // [nv] g_ref, h_ref
-    const float2 *fref_f2;
+    gptr<const double4> vel_d;  // [nv] x, y, z, w
+    gptr<const float4> vel_f;   // [nv] x, y, z, |xi|^2
+    gptr<const double2> fref_d2; // [nv] g_ref, h_ref
+    gptr<const float2> fref_f2;
     // axis tables of tensor-product velocity sets (nt == 0: none)
     int nt;
-    const int *vtab;            // [nv] packed offsets of a velocity's axis entries
-    const double *tab_x;        // [nt] abscissa of each entry
-    const int *tab_ax;          // [nt] axis of each entry
-    const double *tab_w;        // [nt] 1D weight of each entry (w(xi) = prod of its axis weights), or null
-    void *face_tab;             // [nf*nt] per-face axis factors (storage type)
+    gptr<const int> vtab;       // [nv] packed offsets of a velocity's axis entries
+    gptr<const double> tab_x;   // [nt] abscissa of each entry
+    gptr<const int> tab_ax;     // [nt] axis of each entry
+    gptr<const double> tab_w;   // [nt] 1D weight of each entry (w(xi) = prod of its axis weights), or null
+    gptr<void> face_tab;        // [nf*nt] per-face axis factors (storage type)
     // gather stencils: group = centre cell + its LSQ neighbours (padded with
     // the centre and zero weights); cells: groups of the cell and of the cell
     // across each face; faces: groups of owner and neighbour
-    const int *cell_list;       // interior cells, then cells with a boundary face
-    const int *st_id;           // [nc][(1+MAXF)*(1+MAXN)]
-    const double *st_a;         // [nc][(1+MAXF)*MAXN*3]
-    const float *st_af;
-    const int *fs_id;           // [nf][2*(1+MAXN)]
-    const double *fs_a;         // [nf][2*MAXN*3]
-    const float *fs_af;
-    const int *bnd_face;        // [nb]
-    const int *bnd_mark;        // [nb] index into marks
-    const void *marks;          // meso_mark_desc[] (host layout)
+    gptr<const int> cell_list;  // interior cells, then cells with a boundary face
+    gptr<const int> st_id;      // [nc][(1+MAXF)*(1+MAXN)]
+    gptr<const double> st_a;    // [nc][(1+MAXF)*MAXN*3]
+    gptr<const float> st_af;
+    gptr<const int> fs_id;      // [nf][2*(1+MAXN)]
+    gptr<const double> fs_a;    // [nf][2*MAXN*3]
+    gptr<const float> fs_af;
+    gptr<const int> bnd_face;   // [nb]
+    gptr<const int> bnd_mark;   // [nb] index into marks
+    gptr<const void> marks;     // meso_mark_desc[] (host layout)
     // dvs
-    const double *xi;           // [nv*3]
-    const float *xif;
-    const double *w;            // [nv]
-    const int *mirror;          // [nv*3] or null
+    gptr<const double> xi;      // [nv*3]
+    gptr<const float> xif;
+    gptr<const double> w;       // [nv]
+    gptr<const int> mirror;     // [nv*3] or null
     // state
-    const void *fin_g, *fin_h;  // cell-major, f^n
-    void *fout_g, *fout_h;      // cell-major, f^{n+1}
+    gptr<const void> fin_g, fin_h; // cell-major, f^n
+    gptr<void> fout_g, fout_h;  // cell-major, f^{n+1}
     // FP32X storage: f^n / f^{n+1} hold FP32 words of the deviation from f_ref
     // and these arrays the extension byte of every word (same indexing); null
     // for FP32 and FP64 storage
-    const unsigned char *xin_g, *xin_h;
-    unsigned char *xout_g, *xout_h;
-    const double *rho_in, *u_in, *T_in, *q_in;
-    double *rho_out, *u_out, *T_out, *q_out;
+    gptr<const unsigned char> xin_g, xin_h;
+    gptr<unsigned char> xout_g, xout_h;
+    gptr<const double> rho_in, u_in, T_in, q_in;
+    gptr<double> rho_out, u_out, T_out, q_out;
     // scratch
-    double *part_f;             // [nf][ntile_f][ncol]
-    EqParams *face_eq;          // [nf]
-    double *face_flux;          // [nf][D+2] flux moments along n0 of the final face value
-    BndParams *bnd;             // [nb]
-    double *part_c;             // [nc][ntile_c][4]
-    double *fm;                 // [nc][5] cell flux moments: m0, m1 (3), m2
+    gptr<double> part_f;        // [nf][ntile_f][ncol]
+    gptr<EqParams> face_eq;     // [nf]
+    gptr<double> face_flux;     // [nf][D+2] flux moments along n0 of the final face value
+    gptr<BndParams> bnd;        // [nb]
+    gptr<double> part_c;        // [nc][ntile_c][4]
+    gptr<double> fm;            // [nc][5] cell flux moments: m0, m1 (3), m2
     // control
-    int *bad_count;
-    int *first_bad;
-    int *bad_step;
-    int *halt;
-    int *step_ctr;              // steps since the initial state (device counter)
-    const int *mask;            // [nc] 1: a bad cell is repaired (Shakhov); null: no masked group
-    int *masked;                // bad cells repaired so far (device counter)
+    gptr<int> bad_count;
+    gptr<int> first_bad;
+    gptr<int> bad_step;
+    gptr<int> halt;
+    gptr<int> step_ctr;         // steps since the initial state (device counter)
+    gptr<const int> mask;       // [nc] 1: a bad cell is repaired (Shakhov); null: no masked group
+    gptr<int> masked;           // bad cells repaired so far (device counter)
     // LDS-staged patch kernels (2D): update patches (interior cells) and
     // moment patches (all faces); the velocity range is split into p_chunks
     // chunks of p_vchunk velocities
     int n_up, up_maxi, up_maxf, up_maxo, up_maxr;
-    const void *up_items;         // meso_patch::ItemT<D>
-    const void *up_faces;         // meso_patch::FaceT<D>
-    const int *up_item_ptr, *up_face_ptr, *up_nown, *up_row_ptr, *up_rows;
+    gptr<const void> up_items;    // meso_patch::ItemT<D>
+    gptr<const void> up_faces;    // meso_patch::FaceT<D>
+    gptr<const int> up_item_ptr, up_face_ptr, up_nown, up_row_ptr, up_rows;
     int n_mp, mp_maxi, mp_maxf, mp_maxr;
-    const void *mp_items;
-    const void *mp_faces;
-    const int *mp_item_ptr, *mp_face_ptr, *mp_row_ptr, *mp_rows;
-    const int *mp_nown;         // cell-based moment patches: owned cells per patch
+    gptr<const void> mp_items;
+    gptr<const void> mp_faces;
+    gptr<const int> mp_item_ptr, mp_face_ptr, mp_row_ptr, mp_rows;
+    gptr<const int> mp_nown;    // cell-based moment patches: owned cells per patch
     int p_chunks, p_vchunk;     // velocity chunks of the patch kernels and their size (a multiple of 64)
     int n_int_cells;            // interior cells: cell_list[0, n_int_cells)
-    void *crec;                 // [nc] CellRec of the storage type
+    gptr<void> crec;            // [nc] CellRec of the storage type
     // patch kernels: f-bar+ of every face, written by the moment kernel and
     // read by the update kernel (face-major [nf][nvp], storage type)
-    void *fbar_g, *fbar_h;
-    // FP32 storage: f-bar+ of the boundary faces unrounded (FP64 deviation
-    // from f_ref), [boundary face][nvp], for k_boundary; null otherwise
-    double *fbnd_g, *fbnd_h;
-    void *cell_tab;             // [nc][2][nt] axis tables of the cell equilibria
+    gptr<void> fbar_g, fbar_h;
+    gptr<void> cell_tab;        // [nc][2][nt] axis tables of the cell equilibria
     // FP32 storage keeps f - f_ref; f_ref is the equilibrium `ref` at rest.
     EqParams ref;
-    const double *fref_d;  // [2][nv] g_ref then h_ref (FP64)
-    const float *fref_f;   // same, FP32
+    gptr<const double> fref_d; // [2][nv] g_ref then h_ref (FP64)
+    gptr<const float> fref_f; // same, FP32
     double mref[17];       // raw face moments of f_ref (face-moment column layout)
     double rflux[5][3];    // per flux column k: sum_v w xi psi_k f_ref
 };
+
+// A context's arguments from a device table (the fused batch kernels).
+__device__ __forceinline__ KArgs batch_args(const KArgs *__restrict__ tab, int k) { return tab[k]; }
 
 // Equilibrium in the stored form: the full value for FP64 storage, the
 // deviation from f_ref for FP32 storage.

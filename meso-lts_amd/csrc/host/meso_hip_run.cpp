@@ -87,6 +87,10 @@ class HipSolver {
         m0_.assign(nc_, 0.0);
         m1_.assign(3 * size_t(nc_), 0.0);
         m2_.assign(nc_, 0.0);
+        // cell groups with `mask switch True` (src/solver/boundary/patch.cpp:48-51)
+        std::vector<int32_t> mask(nc_, 0);
+        check(meso_case_cell_mask(cs_, mask.data()), meso_case_last_error());
+        check(meso_hip_set_mask(ctx_, mask.data()), meso_hip_last_error(ctx_));
     }
     ~HipSolver() {
         meso_hip_destroy(ctx_);
@@ -129,13 +133,20 @@ class HipSolver {
     void do_step() {
         meso_step_report rep{};
         const int rc = meso_hip_step(ctx_, 1, &rep);
+        // cells of masked groups were repaired by the step (inverse-distance
+        // average of the neighbours, cdugks_shakhov.cpp:698-719, 766-789)
+        if (rep.masked_cells > 0)
+            std::cerr << "[WARN] step " << step + 1 << ": " << rep.masked_cells
+                      << " masked cell(s) caught rho=nan or T < 0 and were repaired" << std::endl;
         if (rc == MESO_E_NUMERIC) {
-            // meso masks such cells by IDW interpolation (cdugks_shakhov.cpp:698-719,
-            // 766-789); here the run stops with the state of the last good step
-            std::cerr << "[Warn] bad cell " << rep.first_bad_cell + 1 << " (NaN density or T < 0) at step "
+            // an unmasked cell: meso logs it and ends the run (run_state = false)
+            // after completing the step, whose state is written as the final output
+            std::cerr << "[WARN] cell<" << rep.first_bad_cell + 1 << "> caught rho=nan or T < 0 at step "
                       << step + 1 << std::endl;
             failed_ = true;
             run_state_ = false;
+            ++step;
+            solution_time += dt_;
             return;
         }
         check(rc, meso_hip_last_error(ctx_));

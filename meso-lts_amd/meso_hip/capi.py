@@ -57,7 +57,8 @@ class StepParams(C.Structure):
 class TileConfig(C.Structure):
     _fields_ = [("patches", C.c_int32), ("update_cells", C.c_int32), ("moment_by_cells", C.c_int32),
                 ("moment_size", C.c_int32), ("vchunk", C.c_int32), ("update_vt", C.c_int32),
-                ("moment_vt", C.c_int32), ("update_owned", C.c_int32)]
+                ("moment_vt", C.c_int32), ("update_owned", C.c_int32),
+                ("moment_lds", C.c_int32), ("update_lds", C.c_int32)]
 
 
 class StepReport(C.Structure):
@@ -108,6 +109,14 @@ def lib() -> C.CDLL:
             "meso_hip_get_state": ([vp, _d, C.c_int32], C.c_int),
             "meso_hip_step": ([vp, C.c_int32, C.POINTER(StepReport)], C.c_int),
             "meso_hip_step_batch": ([C.POINTER(vp), C.c_int32, C.c_int32, C.POINTER(StepReport)], C.c_int),
+            "meso_hip_batch_create": ([C.POINTER(vp), C.c_int32, C.POINTER(vp)], C.c_int),
+            "meso_hip_batch_step": ([vp, C.c_int32, C.POINTER(StepReport)], C.c_int),
+            "meso_hip_batch_set_graphs": ([vp, C.c_int32], C.c_int),
+            "meso_hip_batch_set_rechunk": ([vp, C.c_int32], C.c_int),
+            "meso_hip_batch_set_streams": ([vp, C.c_int32], C.c_int),
+            "meso_hip_set_vchunk": ([vp, C.c_int32], C.c_int),
+            "meso_hip_batch_info": ([vp, _i, _i, _i], C.c_int),
+            "meso_hip_batch_destroy": ([vp], None),
             "meso_hip_autotune": ([vp, C.c_int32, _i], C.c_int),
             "meso_hip_set_graphs": ([vp, C.c_int32], C.c_int),
             "meso_hip_tiles": ([vp, C.POINTER(TileConfig)], C.c_int),
@@ -349,6 +358,10 @@ class Device:
         """Replay captured step pairs (meso_hip_set_graphs)."""
         self._check(lib().meso_hip_set_graphs(self._h, 1 if enable else 0))
 
+    def set_vchunk(self, vchunk: int):
+        """Velocities per block of the patch kernels (meso_hip_set_vchunk; 0: default)."""
+        self._check(lib().meso_hip_set_vchunk(self._h, int(vchunk)))
+
     def set_params(self, p: StepParams):
         self._check(lib().meso_hip_set_params(self._h, C.byref(p)))
 
@@ -399,3 +412,64 @@ def step_batch(devices, n=1):
     if rc not in (MESO_OK, MESO_E_NUMERIC):
         raise MesoError(rc, lib().meso_hip_last_error(None).decode())
     return list(reps)
+
+
+class Batch:
+    """A fixed set of Devices advanced with one launch per stage
+    (meso_hip_batch_*): the devices on the patch kernels share each stage's
+    launch; the others keep their own launches in the same call."""
+
+    def __init__(self, devices):
+        self.devices = list(devices)
+        k = len(self.devices)
+        arr = (C.c_void_p * k)(*[d._h for d in self.devices])
+        h = C.c_void_p()
+        rc = lib().meso_hip_batch_create(arr, k, C.byref(h))
+        if rc != MESO_OK:
+            raise MesoError(rc, lib().meso_hip_last_error(None).decode())
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().meso_hip_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        """Advance every device by n steps; returns the per-device StepReports."""
+        k = len(self.devices)
+        reps = (StepReport * k)()
+        rc = lib().meso_hip_batch_step(self._h, n, reps)
+        if rc not in (MESO_OK, MESO_E_NUMERIC):
+            raise MesoError(rc, lib().meso_hip_last_error(None).decode())
+        return list(reps)
+
+    def set_graphs(self, enable: bool):
+        rc = lib().meso_hip_batch_set_graphs(self._h, 1 if enable else 0)
+        if rc != MESO_OK:
+            raise MesoError(rc, lib().meso_hip_last_error(None).decode())
+
+    def set_streams(self, n: int):
+        """Sub-batches on their own streams (meso_hip_batch_set_streams, default 4)."""
+        rc = lib().meso_hip_batch_set_streams(self._h, int(n))
+        if rc != MESO_OK:
+            raise MesoError(rc, lib().meso_hip_last_error(None).decode())
+
+    def set_rechunk(self, enable: bool):
+        """Let the plan set the fused devices' velocity chunks for the batch grid (default on)."""
+        rc = lib().meso_hip_batch_set_rechunk(self._h, 1 if enable else 0)
+        if rc != MESO_OK:
+            raise MesoError(rc, lib().meso_hip_last_error(None).decode())
+
+    def info(self):
+        """(fused devices, kernel-instance groups, launches per fused step); builds the plan"""
+        f, g, l = C.c_int32(), C.c_int32(), C.c_int32()
+        rc = lib().meso_hip_batch_info(self._h, C.byref(f), C.byref(g), C.byref(l))
+        if rc != MESO_OK:
+            raise MesoError(rc, lib().meso_hip_last_error(None).decode())
+        return f.value, g.value, l.value
