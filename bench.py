@@ -325,6 +325,8 @@ def main():
                     help="instead: the batched cavity sweep of N seeded instances (FP32 storage)")
     ap.add_argument("--batch-streams", type=int, default=4, metavar="S",
                     help="--batch: sub-batches of the fused batch, one stream each (meso_hip_batch_set_streams)")
+    ap.add_argument("--vchunk", type=int, default=0,
+                    help="velocities per block of the patch kernels (meso_hip_set_vchunk; 0: the default)")
     ap.add_argument("--autotune", action="store_true",
                     help="run meso_hip_autotune before timing (off by default: the default tiles are "
                          "deterministic, so that profiles of the same command see the same kernels)")
@@ -370,6 +372,8 @@ def main():
     for storage, tag in kinds:
         dev = capi.Device(cs, storage)
         dev.init_equilibrium()
+        if args.vchunk:
+            dev.set_vchunk(args.vchunk)
         if args.autotune:
             dev.autotune(3)
         tiles[tag] = dev.tiles()

@@ -164,11 +164,20 @@ __device__ __forceinline__ double bnd_mix_full(const KArgs &a, const T *__restri
     const int nvp = a.s.nvp;
     if constexpr (sizeof(T) == 8) {
         // FP64 and FP32X storage with the patch kernels: the moment kernel
-        // wrote this face's f-bar+ (the same reconstruction, FP64); FP32
-        // storage rounds it, so it reconstructs here in FP64
+        // wrote this face's f-bar+ (the same reconstruction, FP64)
         if (a.fbar_g) {
             const T *fbr = k == 0 ? (const T *)a.fbar_g : (const T *)a.fbar_h;
             return (1.0 - mix) * double(fbr[size_t(f) * nvp + v]) + mix * eq_full;
+        }
+    }
+    if constexpr (sizeof(T) == 4) {
+        // FP32 storage with the patch kernels: the moment kernel's f-bar+ (the
+        // FP64 reconstruction, its deviation rounded to FP32), the values the
+        // update kernel uses for the outgoing half of the same face
+        if (a.fbar_g) {
+            const T *fbr = k == 0 ? (const T *)a.fbar_g : (const T *)a.fbar_h;
+            const double fb = double(fbr[size_t(f) * nvp + v]) + a.fref_d[k * a.s.nv + v];
+            return (1.0 - mix) * fb + mix * eq_full;
         }
     }
     // (FP32X storage always runs the patch kernels, so its values come from f-bar+ above)
