@@ -13,8 +13,11 @@
 //      update:   cell flux from the face rows, f-tilde-plus update, q partials
 // All arithmetic of the transport part (gradients, reconstruction, fluxes,
 // the update itself) is FP64; FP32 storage only rounds the stored deviation
-// f - f_ref. The equilibria of FP32 storage are evaluated in FP32 in
-// deviation form (their weight in one step is dt/tau or C_s).
+// f - f_ref and the handed-over face values, both stochastically (unbiased:
+// near a steady state the per-step increments fall below half a unit in the
+// last place, and round-to-nearest drops them and stalls the state). The
+// equilibria of FP32 storage are evaluated in FP32 in deviation form (their
+// weight in one step is dt/tau or C_s).
 //
 // LIM: the Venkatakrishnan limiter of the reconstruction (Shakhov
 // limiter-switch); a template argument, so that the kernels without it keep
@@ -497,6 +500,9 @@ __device__ __forceinline__ void d_patch_moments(const KArgs &a, const int b) {
     for (int k = 0; k < NCOL; ++k) acc[k] = 0.0;
     const int ntiles = (a.s.nv + VT - 1) / VT;
     const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
+    // FP32 storage: seed of the stochastic rounding of the handed-over face
+    // values (the device step counter; its own stream, apart from the state's)
+    const unsigned fseed = sizeof(S) == 4 ? mix32(unsigned(*a.step_ctr) * 0x85ebca6bu + 0x27d4eb2fu) : 0u;
     __syncthreads();
     for (int t = t0; t < t1; ++t) {
         const int v = t * VT + vl;
@@ -629,8 +635,18 @@ __device__ __forceinline__ void d_patch_moments(const KArgs &a, const int b) {
                         // these rounded ones; keeping unrounded copies here
                         // took the 2D kernel from 69 to 96 VGPRs, 3 to 2
                         // blocks per CU, 4.9 to 6.9 ms)
-                        const S fv = S(mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
-                                                                                 : fc[d] + Xd);
+                        const double fvd = mode == meso_patch::SIDE_BOUNDARY && dead ? 0.5 * (fc[d] + fc[d] + Xd + Xd)
+                                                                                     : fc[d] + Xd;
+                        // FP32: stochastically rounded (unbiased): round to
+                        // nearest hands the update the same rounding error
+                        // every step, a bias that shifts the steady state
+                        S fv;
+                        if constexpr (sizeof(S) == 4) {
+                            const unsigned rf = mix32(unsigned(fi) ^ fseed);
+                            fv = f32_sr(fvd, d == 0 ? rf : rot16(rf));
+                        } else {
+                            fv = S(fvd);
+                        }
                         *dst = fv;
                         (d == 0 ? (S *)a.fbar_g : (S *)a.fbar_h)[fi] = fv;
                     } else if (dead) {
@@ -796,8 +812,9 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
     }
     const int ntiles = (a.s.nv + VT - 1) / VT;
     const int t0 = chunk * (a.p_vchunk / VT), t1 = min(t0 + a.p_vchunk / VT, ntiles);
-    // XS: seed of the stochastic rounding of the stored values (the device step counter)
-    const unsigned seed = XS ? mix32(unsigned(*a.step_ctr) * 0x9e3779b9u + 0x632be5abu) : 0u;
+    // FP32 and FP32X: seed of the stochastic rounding of the stored values (the
+    // device step counter)
+    const unsigned seed = XS || sizeof(S) == 4 ? mix32(unsigned(*a.step_ctr) * 0x9e3779b9u + 0x632be5abu) : 0u;
     // Shakhov heat-flux partials of this thread's owned cells over the chunk
     constexpr int UPD_QJ = upd_qj(VT);
     double qreg[UPD_QJ][D];
@@ -1015,6 +1032,11 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
                         ((float *)fout_h)[o] = wd;
                         a.xout_h[o] = e;
                     }
+                } else if constexpr (sizeof(S) == 4) {
+                    // FP32: the stored deviation stochastically rounded
+                    const unsigned rg = mix32(unsigned(o) ^ seed);
+                    fout_g[o] = f32_sr(og, rg);
+                    if (SH) fout_h[o] = f32_sr(oh, rot16(rg));
                 } else {
                     fout_g[o] = S(og);
                     if (SH) fout_h[o] = S(oh);

@@ -61,6 +61,18 @@ __host__ __device__ __forceinline__ unsigned mix32(unsigned x) {
     x ^= x >> 16;
     return x;
 }
+// FP32 with stochastic rounding: r holds 29 random bits, added to the FP64
+// significand below its 23 kept bits before truncating, so that v rounds up
+// with the probability of its position between its two FP32 neighbours
+// (unbiased; the stored state then does not stall near a steady state the way
+// round-to-nearest makes it). Non-finite values are converted as they are.
+__device__ __forceinline__ float f32_sr(double v, unsigned r) {
+    if (!(fabs(v) <= 1.0e300)) return float(v);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v) + (r & ((1u << 29) - 1u));
+    return float(__longlong_as_double((long long)(b & ~((1ull << 29) - 1))));
+}
+// Second random word from a first (the other distribution of the same entry).
+__device__ __forceinline__ unsigned rot16(unsigned x) { return (x << 16) | (x >> 16); }
 __device__ __forceinline__ double x32_decode(float hi, unsigned ext) {
     const unsigned long long b = (unsigned long long)__double_as_longlong(double(hi));
     return __longlong_as_double((long long)(b | ((unsigned long long)(ext & 0xffu) << 21)));
