@@ -287,8 +287,9 @@ __device__ __forceinline__ void bnd_value_full(const KArgs &a, const BndParams &
 // A bad cell (NaN density, T < 0 or NaN T) after the conservative update. A
 // Shakhov cell of a masked group is marked (mask 2) for k_mask_repair and
 // counted; any other stops the run after this step (cdugks_shakhov.cpp:698-719).
+// step_off: how far the device step counter has run ahead of this step.
 template <int MODEL>
-__device__ __forceinline__ void flag_bad(const KArgs &a, int c) {
+__device__ __forceinline__ void flag_bad(const KArgs &a, int c, int step_off = 0) {
     if (MODEL == CDUGKS_SHAKHOV && a.mask && a.mask[c]) {
         const_cast<int *>(static_cast<const int *>(a.mask))[c] = 2;
         atomicAdd(a.masked, 1);
@@ -296,7 +297,7 @@ __device__ __forceinline__ void flag_bad(const KArgs &a, int c) {
     }
     atomicAdd(a.bad_count, 1);
     atomicMin(a.first_bad, c);
-    atomicMin(a.bad_step, *a.step_ctr);
+    atomicMin(a.bad_step, *a.step_ctr - step_off);
 }
 
 // Sign of n0 . xi against the +-VSMALL dead-band (meso's upwind test,
@@ -586,6 +587,65 @@ __device__ __forceinline__ void shakhov_flux_sep(const EqParams &e, const double
     out[D + 1] = e.A * (e2 + 2.0 * ud + uu * sg0 + sh0);
 }
 
+// Face macros from the raw moments M of f-bar+ (face-moment column layout):
+// rho, u, T, the Shakhov heat flux (cdugks_shakhov.cpp:317-362; BGK
+// cdugks.cpp:146-170), the equilibrium constants e with its mix weight, and Fb,
+// the flux moments of f-bar+ itself along n.
+template <int MODEL, int D>
+__device__ __forceinline__ void face_macros(const KArgs &a, const double *M, const double n[3], EqParams &e,
+                                            double *Fb) {
+    const double rho = M[Col<D>::M0];
+    const double inv = 1.0 / rho;
+    double u[3] = {M[Col<D>::M1] * inv, M[Col<D>::M1 + 1] * inv, D == 3 ? M[Col<D>::M1 + 2] * inv : 0.0};
+    double M1[3] = {M[Col<D>::M1], M[Col<D>::M1 + 1], D == 3 ? M[Col<D>::M1 + 2] : 0.0};
+    double Mn[3];  // M2 . n
+    for (int i = 0; i < 3; ++i) {
+        Mn[i] = 0.0;
+        if (i < D)
+            for (int j = 0; j < D; ++j) Mn[i] += m2_at<D>(M, i, j) * n[j];
+    }
+    Fb[0] = n[0] * M1[0] + n[1] * M1[1] + n[2] * M1[2];
+    for (int i = 0; i < D; ++i) Fb[1 + i] = Mn[i];
+    const double hdt = a.s.half_dt;
+    if (MODEL == CDUGKS_SHAKHOV) {
+        double trM2 = 0.0;
+        for (int i = 0; i < D; ++i) trM2 += m2_at<D>(M, i, i);
+        const double H0 = M[Col<D>::H0];
+        const double rhoE = 0.5 * (trM2 + H0);
+        const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const double T_ = (rhoE / rho - 0.5 * uu) / a.s.Cv;
+        const double tau = tau_of(a.s, rho, T_);
+        // central heat flux sum w c (|c|^2 g + h), c = xi - u, from raw moments
+        double Mu[3];
+        for (int i = 0; i < 3; ++i) {
+            Mu[i] = 0.0;
+            if (i < D)
+                for (int j = 0; j < D; ++j) Mu[i] += m2_at<D>(M, i, j) * u[j];
+        }
+        const double M1u = M1[0] * u[0] + M1[1] * u[1] + M1[2] * u[2];
+        double q[3];
+        const double cq = tau / (2.0 * tau + hdt * a.s.Pr);
+        for (int i = 0; i < 3; ++i) {
+            const double M3i = i < D ? M[Col<D>::M3 + i] : 0.0;
+            const double H1i = i < D ? M[Col<D>::H1 + i] : 0.0;
+            const double m3 = M3i - 2.0 * Mu[i] + uu * M1[i] - u[i] * trM2 + 2.0 * u[i] * M1u -
+                              u[i] * uu * rho + H1i - u[i] * H0;
+            q[i] = cq * m3;
+        }
+        make_shakhov(e, rho, u, T_, q, a.s.R, a.s.Pr, D, a.s.K);
+        e.mix = hdt / (2.0 * tau + hdt);
+        double M3n = 0.0, H1n = 0.0;
+        for (int i = 0; i < D; ++i) {
+            M3n += n[i] * M[Col<D>::M3 + i];
+            H1n += n[i] * M[Col<D>::H1 + i];
+        }
+        Fb[D + 1] = M3n + H1n;
+    } else {
+        make_bgk(e, rho, u, a.s.RT);
+        e.mix = hdt / (2.0 * a.s.tau + hdt);
+    }
+}
+
 // (2) face macros, equilibrium constants, axis tables and flux moments of the
 // mixed value of interior faces; one wave per face
 // b: the block's logical index in its context's grid (XCD-mapped, xcd_block).
@@ -622,57 +682,10 @@ __device__ __forceinline__ void d_face_eq(const KArgs &a, const int b) {
             for (int k = 0; k < NCOL; ++k) M[k] += a.mref[k];
         }
         for (int i = 0; i < 3; ++i) n[i] = a.face_n[3 * f + i];
-        const double rho = M[Col<D>::M0];
-        const double inv = 1.0 / rho;
-        double u[3] = {M[Col<D>::M1] * inv, M[Col<D>::M1 + 1] * inv, D == 3 ? M[Col<D>::M1 + 2] * inv : 0.0};
-        double M1[3] = {M[Col<D>::M1], M[Col<D>::M1 + 1], D == 3 ? M[Col<D>::M1 + 2] : 0.0};
-        double Mn[3];  // M2 . n
-        for (int i = 0; i < 3; ++i) {
-            Mn[i] = 0.0;
-            if (i < D)
-                for (int j = 0; j < D; ++j) Mn[i] += m2_at<D>(M, i, j) * n[j];
-        }
-        Fb[0] = n[0] * M1[0] + n[1] * M1[1] + n[2] * M1[2];
-        for (int i = 0; i < D; ++i) Fb[1 + i] = Mn[i];
-        const double hdt = a.s.half_dt;
-        if (MODEL == CDUGKS_SHAKHOV) {
-            double trM2 = 0.0;
-            for (int i = 0; i < D; ++i) trM2 += m2_at<D>(M, i, i);
-            const double H0 = M[Col<D>::H0];
-            const double rhoE = 0.5 * (trM2 + H0);
-            const double uu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-            const double T_ = (rhoE / rho - 0.5 * uu) / a.s.Cv;
-            const double tau = tau_of(a.s, rho, T_);
-            // central heat flux sum w c (|c|^2 g + h), c = xi - u, from raw moments
-            double Mu[3];
-            for (int i = 0; i < 3; ++i) {
-                Mu[i] = 0.0;
-                if (i < D)
-                    for (int j = 0; j < D; ++j) Mu[i] += m2_at<D>(M, i, j) * u[j];
-            }
-            const double M1u = M1[0] * u[0] + M1[1] * u[1] + M1[2] * u[2];
-            double q[3];
-            const double cq = tau / (2.0 * tau + hdt * a.s.Pr);
-            for (int i = 0; i < 3; ++i) {
-                const double M3i = i < D ? M[Col<D>::M3 + i] : 0.0;
-                const double H1i = i < D ? M[Col<D>::H1 + i] : 0.0;
-                const double m3 = M3i - 2.0 * Mu[i] + uu * M1[i] - u[i] * trM2 + 2.0 * u[i] * M1u -
-                                  u[i] * uu * rho + H1i - u[i] * H0;
-                q[i] = cq * m3;
-            }
-            make_shakhov(e, rho, u, T_, q, a.s.R, a.s.Pr, D, a.s.K);
-            if constexpr (sizeof(T) == 4) relate_shakhov(e, a.ref);
-            e.mix = hdt / (2.0 * tau + hdt);
-            double M3n = 0.0, H1n = 0.0;
-            for (int i = 0; i < D; ++i) {
-                M3n += n[i] * M[Col<D>::M3 + i];
-                H1n += n[i] * M[Col<D>::H1 + i];
-            }
-            Fb[D + 1] = M3n + H1n;
-        } else {
-            make_bgk(e, rho, u, a.s.RT);
-            if constexpr (sizeof(T) == 4) relate_bgk(e, a.ref);
-            e.mix = hdt / (2.0 * a.s.tau + hdt);
+        face_macros<MODEL, D>(a, M, n, e, Fb);
+        if constexpr (sizeof(T) == 4) {
+            if (MODEL == CDUGKS_SHAKHOV) relate_shakhov(e, a.ref);
+            else relate_bgk(e, a.ref);
         }
         if constexpr (sizeof(T) == 4) {
             ef = to_eqf(e);
@@ -805,18 +818,13 @@ __device__ __forceinline__ void block_sum(double (&x)[N], double *lds /* [N * BL
     __syncthreads();
 }
 
-// (3) boundary faces: one block per face
-template <int MODEL, int D, class T, bool LIM>
-__device__ __forceinline__ void d_boundary(const KArgs &a, const int b) {
-    if (*a.halt) return;
-    constexpr int NFL = D + 2;
-    __shared__ double lds[(NFL + 2) * (BLOCK / 64)];
-    const int f = a.bnd_face[b];
-    const int c = a.face_cells[f].x;
-    const meso_mark_desc &mk = mark_of(a, a.bnd_mark[b]);
-    const EqParams ef = a.face_eq[f];
-    const int nv = a.s.nv;
-    BndParams P;
+// Boundary face set-up (cdugks_shakhov.cpp:364-607, cdugks.cpp:194-238): the
+// kind as this model handles it and, for every kind but walls, the imposed
+// equilibrium P.in. Returns true for walls and slip walls, whose P.in needs
+// the wall density sums; e1 is then the wall Maxwellian at unit density.
+template <int MODEL, int D>
+__device__ __forceinline__ bool bnd_begin(const KArgs &a, const meso_mark_desc &mk, int f, int c, BndParams &P,
+                                          EqParams &e1) {
     P.kind = mk.type;
     P.dir = mk.direction;
     P.alpha_u = mk.alpha_u;
@@ -864,30 +872,65 @@ __device__ __forceinline__ void d_boundary(const KArgs &a, const int b) {
             double Tw = 0.0;
             if (mk.T_kind == MESO_PATCH_ZERO_GRADIENT) Tw = Tm;
             else if (mk.T_kind == MESO_PATCH_FIXED_VALUE) Tw = mk.temperature;
-            EqParams e1;
             if (MODEL == CDUGKS_SHAKHOV) make_shakhov(e1, 1.0, uw, Tw, zero, R, Pr, D, K);
             else make_bgk(e1, 1.0, uw, a.s.RT);
-            double s[2] = {0.0, 0.0};
-            for (int v = threadIdx.x; v < nv; v += BLOCK) {
-                const double kn = -ndot(a, f, v);
-                if (kn >= VSMALL) {
-                    const double X[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
-                    double gm, hm;
-                    if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<double>(e1, X, D, K, gm, hm);
-                    else gm = bgk_eq<double>(e1, X);
-                    s[1] += kn * a.w[v] * gm;
-                } else {
-                    double gmx, hmx;
-                    bnd_mix_pair_full<MODEL, D, T, LIM>(a, f, c, v, ef, gmx, hmx);
-                    s[0] -= kn * a.w[v] * gmx;
-                }
-            }
-            block_sum<2>(s, lds);
-            const double rho_w = s[0] / s[1];
-            if (MODEL == CDUGKS_SHAKHOV) make_shakhov(P.in, rho_w, uw, mk.temperature, zero, R, Pr, D, K);
-            else make_bgk(P.in, rho_w, uw, a.s.RT);
-        } break;
+            return true;
+        }
         default: break;
+    }
+    return false;
+}
+
+// Terms of the wall density sums at velocity v (kn = xi . n1, n1 into the
+// domain): s1 gathers the unit-density wall Maxwellian over incoming
+// velocities, s0 the mixed face value over outgoing ones.
+template <int MODEL, int D>
+__device__ __forceinline__ void wall_terms(const KArgs &a, const EqParams &e1, int v, double kn, double gmix,
+                                           double s[2]) {
+    if (kn >= VSMALL) {
+        const double X[3] = {a.xi[3 * v], a.xi[3 * v + 1], a.xi[3 * v + 2]};
+        double gm, hm;
+        if (MODEL == CDUGKS_SHAKHOV) shakhov_eq<double>(e1, X, D, a.s.K, gm, hm);
+        else gm = bgk_eq<double>(e1, X);
+        s[1] += kn * a.w[v] * gm;
+    } else {
+        s[0] -= kn * a.w[v] * gmix;
+    }
+}
+
+// P.in of a wall or slip wall from the summed wall density terms.
+template <int MODEL, int D>
+__device__ __forceinline__ void bnd_wall_end(const KArgs &a, const meso_mark_desc &mk, BndParams &P, const double s[2]) {
+    const double rho_w = s[0] / s[1];
+    const double uw[3] = {mk.velocity[0], mk.velocity[1], mk.velocity[2]};
+    const double zero[3] = {0.0, 0.0, 0.0};
+    if (MODEL == CDUGKS_SHAKHOV) make_shakhov(P.in, rho_w, uw, mk.temperature, zero, a.s.R, a.s.Pr, D, a.s.K);
+    else make_bgk(P.in, rho_w, uw, a.s.RT);
+}
+
+// (3) boundary faces: one block per face
+template <int MODEL, int D, class T, bool LIM>
+__device__ __forceinline__ void d_boundary(const KArgs &a, const int b) {
+    if (*a.halt) return;
+    constexpr int NFL = D + 2;
+    __shared__ double lds[(NFL + 2) * (BLOCK / 64)];
+    const int f = a.bnd_face[b];
+    const int c = a.face_cells[f].x;
+    const meso_mark_desc &mk = mark_of(a, a.bnd_mark[b]);
+    const EqParams ef = a.face_eq[f];
+    const int nv = a.s.nv;
+    BndParams P;
+    EqParams e1;
+    if (bnd_begin<MODEL, D>(a, mk, f, c, P, e1)) {
+        double s[2] = {0.0, 0.0};
+        for (int v = threadIdx.x; v < nv; v += BLOCK) {
+            const double kn = -ndot(a, f, v);
+            double gmx = 0.0, hmx;
+            if (!(kn >= VSMALL)) bnd_mix_pair_full<MODEL, D, T, LIM>(a, f, c, v, ef, gmx, hmx);
+            wall_terms<MODEL, D>(a, e1, v, kn, gmx, s);
+        }
+        block_sum<2>(s, lds);
+        bnd_wall_end<MODEL, D>(a, mk, P, s);
     }
     if constexpr (sizeof(T) == 4) {
         if (P.kind != BC_SYMMETRY && P.kind != BC_INTERIOR) {
@@ -1687,6 +1730,7 @@ __global__ void kb_step_end(int *const *__restrict__ ctrs, int n) {
 }
 
 #include "patch_kernels.h"
+#include "small_kernels.h"
 
 }  // namespace
 
@@ -1769,6 +1813,10 @@ struct meso_hip_ctx {
     int n_up = 0, up_maxi = 0, up_maxf = 0, up_maxo = 0, up_maxr = 0, n_mp = 0, mp_maxi = 0, mp_maxf = 0,
         mp_maxr = 0;
     int p_chunks = 0, p_vchunk = 0;
+    // small velocity sets (k_small_faces / k_small_cells): lanes per face or
+    // cell (16, 32 or 64), 0 when the context has no such step
+    int small_lp = 0;
+    double *sflux = nullptr;  // [nd][nf][small_lp]
     int p_chunk_cap = 0;        // chunks the part_f / part_c buffers hold (>= p_chunks)
     // captured step pairs (meso_hip_set_graphs): gexec[p] runs two steps from
     // buffer parity p; valid while gver[p] == kver, which changes with every
@@ -1991,6 +2039,7 @@ KArgs make_args(meso_hip_ctx *c) {
     a.bnd = c->bnd;
     a.part_c = c->part_c;
     a.fm = c->fm;
+    a.sflux = c->sflux;
     a.bad_count = c->flags;
     a.first_bad = c->flags + 1;
     a.bad_step = c->flags + 2;
@@ -2238,8 +2287,35 @@ int step_impl(meso_hip_ctx *c, KArgs a) {
     return step_body<MODEL, D, T, TAB, false>(c, a);
 }
 
+// Whether the context's step is the small-velocity-set step (two launches;
+// the face kernel advances the step counter).
+bool uses_small(const meso_hip_ctx *c) { return c->small_lp > 0 && !c->s.limiter; }
+
+template <int MODEL, int D, int LP>
+int step_small(meso_hip_ctx *c, KArgs a) {
+    constexpr int SPB = small_spb<LP>();
+    LAUNCH(c, "small_faces", (k_small_faces<MODEL, D, LP>), (c->s.nf + SPB - 1) / SPB, a);
+    LAUNCH(c, "small_cells", (k_small_cells<MODEL, D, LP>), (c->s.nc + SPB - 1) / SPB, a);
+    if (MODEL == CDUGKS_SHAKHOV && c->has_mask) LAUNCH(c, "mask_repair", (k_mask_repair<D, double>), 1, a);
+    HIP_TRY(c, hipGetLastError());
+    return MESO_OK;
+}
+
+template <int MODEL, int D>
+int step_small_lp(meso_hip_ctx *c, KArgs a) {
+    switch (c->small_lp) {
+        case 16: return step_small<MODEL, D, 16>(c, a);
+        case 32: return step_small<MODEL, D, 32>(c, a);
+        default: return step_small<MODEL, D, 64>(c, a);
+    }
+}
+
 int step_dispatch(meso_hip_ctx *c, KArgs a) {
     const int m = c->s.model, D = c->s.D;
+    if (uses_small(c)) {
+        if (m == CDUGKS_SHAKHOV) return D == 2 ? step_small_lp<CDUGKS_SHAKHOV, 2>(c, a) : step_small_lp<CDUGKS_SHAKHOV, 3>(c, a);
+        if (m == CDUGKS_BGK) return D == 2 ? step_small_lp<CDUGKS_BGK, 2>(c, a) : step_small_lp<CDUGKS_BGK, 3>(c, a);
+    }
     const bool f32 = c->storage == MESO_STORAGE_FP32;
     const bool tab = c->nt > 0;
 #define MESO_S(M, DD)                                                                                \
@@ -3019,6 +3095,23 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
             c->fbar[k] = fb;
         }
     }
+    // small velocity sets: the two-launch step (k_small_faces, k_small_cells)
+    // for the CDUGKS models with FP64 storage; a Shakhov mirror boundary
+    // (symmetry, slip wall) needs the mirrored velocity's face value and keeps
+    // the per-stage kernels
+    if (p->model != MESO_MODEL_DUGKS_BGK && nv <= 64 && !ext && storage == MESO_STORAGE_FP64 &&
+        env_int("MESO_HIP_SMALL", 1) != 0) {
+        bool mirror = false;
+        for (int f = 0; f < nf && p->model == MESO_MODEL_CDUGKS_SHAKHOV; ++f) {
+            if (c->face_bnd_h[f] < 0) continue;
+            const int kind = m->marks[m->face_mark[f]].type;
+            mirror = mirror || kind == MESO_BC_SYMMETRY || kind == MESO_BC_SLIP_WALL;
+        }
+        if (!mirror) {
+            c->small_lp = nv <= 16 ? 16 : (nv <= 32 ? 32 : 64);
+            if ((rc = dalloc(c, &c->sflux, size_t(c->nd) * nf * c->small_lp))) return fail(rc);
+        }
+    }
     c->xi_h = xi_s;
     c->w_h = w_s;
     if (ext && (!c->patches || c->n_bcell > 0 || p->model == MESO_MODEL_DUGKS_BGK))
@@ -3285,7 +3378,7 @@ static int step_one(meso_hip_ctx *c) {
     KArgs a = make_args(c);
     const int rc = step_dispatch(c, a);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->flags + 4);
+    if (!uses_small(c)) hipLaunchKernelGGL(k_step_end, dim3(1), dim3(1), 0, c->stream, c->flags + 4);
     HIP_TRY(c, hipGetLastError());
     c->cur = 1 - c->cur;
     return MESO_OK;

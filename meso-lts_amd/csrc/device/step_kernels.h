@@ -134,6 +134,7 @@ struct KArgs {
     gptr<BndParams> bnd;        // [nb]
     gptr<double> part_c;        // [nc][ntile_c][4]
     gptr<double> fm;            // [nc][5] cell flux moments: m0, m1 (3), m2
+    gptr<double> sflux;         // small velocity sets: [nd][nf][LP] (xi . n0) f_face per face and velocity
     // control
     gptr<int> bad_count;
     gptr<int> first_bad;

@@ -76,11 +76,19 @@ def case_factory(tmp_path_factory):
         elif name == "shak_hex3":
             p = mg.shakhov_cavity_case(d, mg.hex_cavity(3, jitter=0.05, seed=2), ("sparse-gh", 4, 0.5),
                                        mg.viscosity_from_kn(0.1), 0.1)
+        elif name == "shak_quad6_nc6":  # 36 velocities: 64-lane segments of the small-set step
+            p = mg.shakhov_cavity_case(d, mg.quad_cavity(6, jitter=0.1, seed=3), ("nc", 1, 5, 4.5), mu075, 0.2)
+        elif name == "shak_tri6_nc4":  # 16 velocities: 16-lane segments
+            p = mg.shakhov_cavity_case(d, mg.tri_cavity(6, jitter=0.2, seed=1), ("nc", 1, 3, 4.5), mu075, 0.2)
+        elif name == "shak_hex3_gh3":  # 27 velocities in 3D: 32-lane segments
+            p = mg.shakhov_cavity_case(d, mg.hex_cavity(3, jitter=0.05, seed=2), ("sparse-gh", 3, 0.5),
+                                       mg.viscosity_from_kn(0.1), 0.1)
         elif name == "shak_kn1_quad10":
             p = mg.shakhov_cavity_case(d, mg.quad_cavity(10, jitter=0.1, seed=7), ("nc", 1, 16, 5.0),
                                        mg.viscosity_from_kn(1.0), 0.2)
         elif name in ("shak_channel_pressure", "shak_channel_farfield", "shak_sym_slip", "bgk_channel_farfield",
-                      "shak_sym_slip_lim"):
+                      "shak_sym_slip_lim", "shak_channel_pressure_nc6", "shak_channel_farfield_nc6",
+                      "shak_sym_slip_nc6"):
             p = channel_case(d, name)
         elif name in ("neumann", "kn0075_cavity", "re400_cavity"):
             # meso's own example cases with its shipped meshes and velocity sets
@@ -113,9 +121,12 @@ def split_sides(m, nx, ny):
 
 def channel_case(d, name):
     """8x4 channel: left/right inlet+outlet, bottom wall, top wall / symmetry / slip
-    (name suffix _lim: with the Venkatakrishnan limiter)."""
+    (name suffix _lim: with the Venkatakrishnan limiter; _nc6: 6x6 Newton-Cotes
+    velocities instead of 12x12)."""
     limiter = name.endswith("_lim")
     name = name[:-4] if limiter else name
+    small = name.endswith("_nc6")
+    name = name[:-4] if small else name
     os.makedirs(d, exist_ok=True)
     m = split_sides(mg.quad_cavity(8, 4, jitter=0.05, seed=11), 8, 4)
     mg.write_neu(m, os.path.join(d, "mesh.neu"))
@@ -131,7 +142,7 @@ def channel_case(d, name):
                  {"name": "bottom", "type": "wall"}, {"name": "top", "type": "wall"}]
         mg.write_case(os.path.join(d, "case.txt"), settings, groups, marks)
         return os.path.join(d, "case.txt")
-    mg.write_newton_cotes(os.path.join(d, "dvs.dvs"), 1, 11, 4.5)
+    mg.write_newton_cotes(os.path.join(d, "dvs.dvs"), 1, 5 if small else 11, 4.5)
     settings = {"solver": "cdugks@shakhov", "mesh-file": "mesh.neu", "dvs-file": "dvs.dvs", "dvs-type": "Newton-Cotes",
                 "Pr": repr(2.0 / 3.0), "timeStep": "1.0e-3", "gas-constant": "0.5", "gas-k": "0",
                 "ref-temperature": "1.0", "ref-viscosity": repr(mg.viscosity_from_kn(0.1)), "vhs-index": "0.5",
