@@ -815,6 +815,7 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
     // FP32 and FP32X: seed of the stochastic rounding of the stored values (the
     // device step counter)
     const unsigned seed = XS || sizeof(S) == 4 ? mix32(unsigned(*a.step_ctr) * 0x9e3779b9u + 0x632be5abu) : 0u;
+    const unsigned fxseed = mix32(seed ^ 0x5bd1e995u);  // FP32: the face flux rows' own stream
     // Shakhov heat-flux partials of this thread's owned cells over the chunk
     constexpr int UPD_QJ = upd_qj(VT);
     double qreg[UPD_QJ][D];
@@ -845,7 +846,11 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
             // Shakhov with axis tables: each face's table entries are read with
             // its f-bar+ values (PF)
             constexpr bool PF = TAB && SH;
-            auto face_flux = [&](const int lf, const S *fbv, const S *tav) {
+            // FP32: random bits of the flux rows' rounding, one hashed word
+            // per lane and tile, stepped by a Weyl increment per face
+            unsigned rq = sizeof(S) == 4 ? mix32(fxseed ^ (unsigned(t) << 20) ^ unsigned(b * BLOCKP + int(threadIdx.x)))
+                                         : 0u;
+            auto face_flux = [&](const int lf, const S *fbv, const S *tav, const unsigned rk) {
                 const Face *F = faces + lf;
                 double sn = 0.0;
 #pragma unroll
@@ -866,9 +871,21 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
                 }
                 if (bnd) bnd_value_lane<MODEL, S, D>(a, f, sn, L, fm[0], fm[1], fm[0], fm[1]);
                 const double knA = sn * F->area;
+                if constexpr (sizeof(S) == 4) {
+                    // FP32: the face flux rows in LDS are stochastically rounded
+                    // as well. Rounded to nearest, their rounding error is a
+                    // fixed function of the state, a bias that moves the steady
+                    // state: on the Kn = 1 cavity at quarter resolution it was
+                    // 57% of the FP32 run's error at convergence (qy 1.36e-6
+                    // against FP64; 5.8e-7 with this rounding)
 #pragma unroll
-                for (int d = 0; d < ND; ++d)
-                    flx[(d * maxf + lf) * VT + vl] = S(dead && !bnd ? 0.0 : knA * fm[d]);
+                    for (int d = 0; d < ND; ++d)
+                        flx[(d * maxf + lf) * VT + vl] = dead && !bnd ? 0.0f : f32_sr(knA * fm[d], d == 0 ? rk : rot16(rk));
+                } else {
+#pragma unroll
+                    for (int d = 0; d < ND; ++d)
+                        flx[(d * maxf + lf) * VT + vl] = S(dead && !bnd ? 0.0 : knA * fm[d]);
+                }
             };
             // FP32: four faces per pass, all four faces' f-bar+ loads (and
             // axis-table entries) issued before any is used (more loads in
@@ -894,7 +911,8 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
                 }
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
-                    if (lf + k * FS < nfp) face_flux(lf + k * FS, fbv[k], tav[k]);
+                    if (lf + k * FS < nfp) face_flux(lf + k * FS, fbv[k], tav[k], rq + unsigned(k) * 0x9e3779b9u);
+                rq += unsigned(NB) * 0x9e3779b9u;
             }
         }
         __syncthreads();

@@ -126,3 +126,40 @@ def test_fp32x_closer_than_fp32(case_factory):
         errs[st] = max(macro_errors(cs, dev.macros(), mo).values())
     _say(f"[gpu] 300 steps: fp32 {errs[capi.STORAGE_FP32]:.2e} fp32x {errs[capi.STORAGE_FP32X]:.2e}")
     assert errs[capi.STORAGE_FP32X] < errs[capi.STORAGE_FP32] / 10
+
+
+def test_fp32_and_fp32x_at_convergence_kn1_cavity(tmp_path):
+    """The Kn = 1 cavity of the benchmark (jittered quads, 101^2 Newton-Cotes on
+    [-5, 5]^2, Shakhov, lid u = 0.2) at quarter resolution (50^2 cells), run in
+    FP64 storage to meso's stopping rule (a residual check every 100 steps; ten
+    checks in a row with every residual <= the case's residual-limit,
+    cdugks_shakhov.cpp:730-756). FP32 and FP32X storage, run for the same steps,
+    are within 1e-6 relative L2 of it on every macro component, the precision
+    target of the FP32 storage (measured: FP32 5.8e-7 in qy, FP32X 2e-9)."""
+    from meso_hip import configs
+    cs = capi.Case(configs.build("cavity_kn1_quad", str(tmp_path), 0.25))
+    limit = cs.setting_double("residual-limit", 1e-6)
+    idx = [0, 4, 1, 2, 5, 6]  # rho, T, u, v, qx, qy of meso's residual vector (2D)
+    _say(f"[gpu] convergence cavity_kn1_quad x0.25 NC={cs.n_cells} NV={cs.n_vel} limit={limit:g}")
+    ref = capi.Device(cs, capi.STORAGE_FP64)
+    ref.init_equilibrium()
+    ref.residual()
+    step, run = 0, 0
+    while run < 10:
+        rep = ref.step(100)
+        assert rep.bad_cells == 0
+        step += 100
+        r = np.asarray(ref.residual())[idx]
+        run = run + 1 if bool(np.all(r <= limit)) else 0
+        assert step <= 20000, "FP64 did not converge"
+    _say(f"[gpu] FP64 converged at step {step}")
+    mr = ref.macros()
+    for storage, tol in ((capi.STORAGE_FP32, 1e-6), (capi.STORAGE_FP32X, 1e-8)):
+        dev = capi.Device(cs, storage)
+        dev.init_equilibrium()
+        rep = dev.step(step)
+        assert rep.bad_cells == 0 and rep.steps_done == step
+        err = macro_errors(cs, dev.macros(), mr)
+        _say(f"[gpu] storage {storage} at step {step}: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()))
+        for k, v in err.items():
+            assert v < tol, f"storage {storage} {k}: {v:.2e}"
