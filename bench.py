@@ -304,6 +304,21 @@ def converge_run(workload, scale, max_steps, time_limit, residual_limit=0.0, cur
     return line
 
 
+def renumbered_case(case_path, how):
+    """A copy of the case directory whose mesh file has its cells renumbered."""
+    import shutil
+    src = os.path.dirname(case_path)
+    dst = f"{src}_{how}"
+    mesh = capi.Case(case_path).setting_str("mesh-file")
+    if not os.path.exists(os.path.join(dst, mesh)):
+        shutil.rmtree(dst, ignore_errors=True)
+        shutil.copytree(src, dst)
+        method = capi.RENUMBER_RCB if how == "rcb" else capi.RENUMBER_RANDOM
+        capi.renumber_mesh(os.path.join(src, mesh), os.path.join(dst, mesh), method, 12345)
+    log(f"[bench] mesh renumbered ({how}): {os.path.join(dst, mesh)}")
+    return os.path.join(dst, os.path.basename(case_path))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,6 +348,9 @@ def main():
                     help="run meso_hip_autotune before timing (off by default: the default tiles are "
                          "deterministic, so that profiles of the same command see the same kernels)")
     ap.add_argument("--graphs", action="store_true", help="replay captured step pairs (meso_hip_set_graphs)")
+    ap.add_argument("--renumber", choices=("rcb", "random"), default=None,
+                    help="run the workload on its mesh file renumbered (meso_mesh_renumber: locality order, or a "
+                         "seeded random order to measure what a poor order costs)")
     ap.add_argument("--converge", action="store_true",
                     help="instead: run FP64, FP32 and FP32X storage to meso's convergence rule and report the "
                          "FP32 / FP32X errors against FP64 there")
@@ -359,6 +377,8 @@ def main():
 
     work = os.path.join(tempfile.gettempdir(), "meso_hip_bench")
     case_path = configs.build(args.workload, work, args.scale)
+    if args.renumber:
+        case_path = renumbered_case(case_path, args.renumber)
     cs = capi.Case(case_path)
     nc, nv = cs.n_cells, cs.n_vel
     shakhov = cs.model == capi.MODEL_CDUGKS_SHAKHOV

@@ -198,6 +198,23 @@ int meso_case_write_output(const meso_case *c, const char *root, int32_t step, d
                            const double *flux_m0, const double *flux_m1_3, const double *flux_m2);
 const char *meso_case_output_error(void);
 
+/* Write the gambit file in_path to out_path with its cells renumbered (node
+ * ids, groups, boundary sets and every other line kept): MESO_RENUMBER_RCB
+ * orders the cells by recursive coordinate bisection of their centres (a
+ * locality order for meshes written in a poor one), MESO_RENUMBER_RANDOM by a
+ * permutation seeded with seed, MESO_RENUMBER_KEEP keeps the order. new_of_old
+ * (may be null; capacity entries) receives each old cell's new 0-based id,
+ * n_cells (may be null) the cell count. No reference counterpart: meso reads
+ * cells in file order (src/mesh/mesh_reader.cpp:52-80) and builds its faces in
+ * that order (src/mesh/object.cpp:38-84); its only mesh utility is the METIS
+ * partition of --mesh (src/mesh/metis.cpp). */
+#define MESO_RENUMBER_RCB 0
+#define MESO_RENUMBER_RANDOM 1
+#define MESO_RENUMBER_KEEP 2
+int meso_mesh_renumber(const char *in_path, const char *out_path, int32_t method, uint64_t seed,
+                       int32_t *new_of_old, int32_t capacity, int32_t *n_cells);
+const char *meso_mesh_renumber_error(void);
+
 /* ======================================================================== */
 /* Device step                                                               */
 /* ======================================================================== */

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <fstream>
 #include <map>
+#include <cstdlib>
 #include <sstream>
 
 namespace MESO {
@@ -42,24 +43,44 @@ int node_num(ObjectType t) {
 }  // namespace Geom
 
 namespace Utils {
+namespace {
+inline bool is_space(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\n' || c == '\v' || c == '\f'; }
+}  // namespace
+// Whitespace-separated tokens (the tokens of an istringstream >> loop).
 StringList split(const String &s) {
-    std::istringstream iss(s);
     StringList out;
-    String tok;
-    while (iss >> tok) {
-        tok.erase(std::remove(tok.begin(), tok.end(), '\r'), tok.end());
-        if (!tok.empty()) out.push_back(tok);
+    const char *p = s.data(), *e = p + s.size();
+    while (p < e) {
+        while (p < e && is_space(*p)) ++p;
+        const char *b = p;
+        while (p < e && !is_space(*p)) ++p;
+        if (p > b) out.emplace_back(b, size_t(p - b));
     }
     return out;
 }
+// Non-empty lines of a text file with '\r' removed (read in one piece: a
+// 640,000-cell gambit file is 76 MB).
 StringList read_lines(const String &file) {
-    std::ifstream fp(file);
+    std::ifstream fp(file, std::ios::binary | std::ios::ate);
     if (!fp.is_open()) throw std::invalid_argument("file open error: " + file);
+    const std::streamsize n = fp.tellg();
+    std::string buf(size_t(std::max<std::streamsize>(n, 0)), '\0');
+    fp.seekg(0);
+    if (n > 0 && !fp.read(&buf[0], n)) throw std::invalid_argument("file read error: " + file);
     StringList lines;
-    String line;
-    while (std::getline(fp, line)) {
-        line.erase(std::remove(line.begin(), line.end(), '\r'), line.end());
-        if (!line.empty()) lines.push_back(line);
+    const char *p = buf.data(), *e = p + buf.size();
+    while (p < e) {
+        const char *b = p;
+        while (p < e && *p != '\n') ++p;
+        const char *q = p;
+        if (p < e) ++p;
+        if (std::find(b, q, '\r') == q) {
+            if (q > b) lines.emplace_back(b, size_t(q - b));
+        } else {
+            String line(b, size_t(q - b));
+            line.erase(std::remove(line.begin(), line.end(), '\r'), line.end());
+            if (!line.empty()) lines.push_back(std::move(line));
+        }
     }
     return lines;
 }
@@ -183,11 +204,30 @@ void Mesh::update_mesh_params() {
     NMARK = int(face_groups.size());
 }
 
+namespace {
+// A face's sorted node ids (at most 4) and kind: the key under which the
+// cells that share the face meet.
+struct FaceKey {
+    std::array<ObjectId, 5> v{{-1, -1, -1, -1, 0}};
+    bool operator==(const FaceKey &o) const { return v == o.v; }
+};
+}  // namespace
+
 void Mesh::generate_face() {
     // Faces are numbered in order of first appearance; the first cell that
     // meets a face owns it. Two cells share a face when its node sets match.
-    std::map<std::vector<ObjectId>, ObjectId> seen;
+    // The faces met so far are listed per smallest node id (a linked list in
+    // flat arrays: head per node, next per face), so a lookup walks the few
+    // faces around one node.
+    ObjectId nmax = 0;
+    for (const auto &cell : cells)
+        for (ObjectId n : cell.node_id) nmax = std::max(nmax, n + 1);
+    std::vector<ObjectId> head(size_t(nmax), -1), next;
+    std::vector<FaceKey> keys;
     faces.clear();
+    faces.reserve(cells.size() * 3);
+    next.reserve(cells.size() * 3);
+    keys.reserve(cells.size() * 3);
     for (auto &cell : cells) {
         const auto &kinds = face_kinds(cell.geom_type);
         const auto &tbl = face_nodes(cell.geom_type);
@@ -195,22 +235,29 @@ void Mesh::generate_face() {
         for (size_t fi = 0; fi < kinds.size(); ++fi) {
             std::vector<ObjectId> nl;
             for (int k : tbl[fi]) nl.push_back(cell.node_id.at(k));
-            std::vector<ObjectId> key = nl;
-            std::sort(key.begin(), key.end());
-            key.push_back(-kinds[fi]);  // keep edge/tria/quad namespaces apart
-            auto it = seen.find(key);
-            if (it == seen.end()) {
+            if (nl.size() > 4) throw std::invalid_argument("<generate_face> a face with more than 4 nodes");
+            FaceKey key;
+            std::copy(nl.begin(), nl.end(), key.v.begin());
+            std::sort(key.v.begin(), key.v.begin() + nl.size());
+            key.v[4] = -kinds[fi];  // keep edge/tria/quad namespaces apart
+            const ObjectId m = key.v[0];
+            if (m < 0) throw std::invalid_argument("<generate_face> negative node id");
+            ObjectId j = head[m];
+            while (j >= 0 && !(keys[j] == key)) j = next[j];
+            if (j < 0) {
                 Face f;
                 f.id = int(faces.size());
                 f.geom_type = kinds[fi];
                 f.node_id = nl;
                 f.cell_id = {cell.id, cell.id};
                 f.cell_face_id = {int(fi), int(fi)};
-                seen.emplace(std::move(key), f.id);
+                keys.push_back(key);
+                next.push_back(head[m]);
+                head[m] = f.id;
                 cell.face_id[fi] = f.id;
                 faces.push_back(std::move(f));
             } else {
-                Face &f = faces[it->second];
+                Face &f = faces[j];
                 f.cell_id[1] = cell.id;
                 f.cell_face_id[1] = int(fi);
                 cell.face_id[fi] = f.id;
@@ -301,7 +348,12 @@ void Mesh::build_geom(double scale) {
 }
 
 namespace {
-int to_int(const String &s) { return std::stoi(s); }
+int to_int(const String &s) {
+    char *end = nullptr;
+    const long v = std::strtol(s.c_str(), &end, 10);
+    if (end == s.c_str()) throw std::invalid_argument("not an integer: " + s);
+    return int(v);
+}
 }
 
 Mesh load_gambit(const String &file_path, double mesh_scale) {

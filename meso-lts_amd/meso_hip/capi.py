@@ -98,6 +98,8 @@ def lib() -> C.CDLL:
             "meso_case_write_output": ([vp, C.c_char_p, C.c_int32, C.c_double, _d, _d, _d, _d, _d, _d, _d],
                                        C.c_int),
             "meso_case_output_error": ([], C.c_char_p),
+            "meso_mesh_renumber": ([C.c_char_p, C.c_char_p, C.c_int32, C.c_uint64, _i, C.c_int32, _i], C.c_int),
+            "meso_mesh_renumber_error": ([], C.c_char_p),
             "meso_hip_create": ([C.POINTER(MeshDesc), C.POINTER(DvsDesc), C.POINTER(StepParams), C.c_int32,
                                  C.POINTER(vp)], C.c_int),
             "meso_hip_destroy": ([vp], None),
@@ -399,6 +401,31 @@ class Device:
 
     def sync(self):
         self._check(lib().meso_hip_sync(self._h))
+
+
+RENUMBER_RCB, RENUMBER_RANDOM, RENUMBER_KEEP = 0, 1, 2
+
+
+def renumber_mesh(in_path: str, out_path: str, method: int = RENUMBER_RCB, seed: int = 0) -> np.ndarray:
+    """Write the gambit file in_path to out_path with its cells renumbered
+    (meso_mesh_renumber: RENUMBER_RCB locality order, RENUMBER_RANDOM seeded
+    permutation, RENUMBER_KEEP file order); returns each old cell's new id."""
+    L = lib()
+    nelem = 0
+    try:
+        with open(in_path) as fp:  # NELEM from the header line after NUMNP
+            for line in fp:
+                if line.split()[:1] == ["NUMNP"]:
+                    nelem = int(next(fp).split()[1])
+                    break
+    except (OSError, ValueError, IndexError, StopIteration):
+        nelem = 0  # the library reports what is wrong with the file
+    perm = np.zeros(nelem, dtype=np.int32)
+    n = C.c_int32(0)
+    rc = L.meso_mesh_renumber(in_path.encode(), out_path.encode(), method, seed, iptr(perm), nelem, C.byref(n))
+    if rc != MESO_OK:
+        raise MesoError(rc, L.meso_mesh_renumber_error().decode())
+    return perm[:n.value]
 
 
 def step_batch(devices, n=1):
