@@ -540,7 +540,11 @@ __device__ __forceinline__ void d_patch_moments(const KArgs &a, const int b) {
         }
         __syncthreads();
         // B: gradients and f-bar+ of the patch faces
-        for (int s = slot; s < nown; s += NSLOT) {
+        // FP32: random bits of the face values' rounding, one hashed word per
+        // lane and tile, stepped by a Weyl increment per (item, face)
+        unsigned rb = sizeof(S) == 4 ? mix32(fseed ^ (unsigned(t) << 20) ^ unsigned(b * BLOCKP + int(threadIdx.x)))
+                                     : 0u;
+        for (int s = slot; s < nown; s += NSLOT, rb += unsigned(meso_patch::Lim<D>::MAXF) * 0x9e3779b9u) {
             const Item *it = rec.item(s);
             const int nface = it->nface;
             double fc[2], g[2][D], mn[2], mx[2];
@@ -642,7 +646,7 @@ __device__ __forceinline__ void d_patch_moments(const KArgs &a, const int b) {
                         // every step, a bias that shifts the steady state
                         S fv;
                         if constexpr (sizeof(S) == 4) {
-                            const unsigned rf = mix32(unsigned(fi) ^ fseed);
+                            const unsigned rf = rb + unsigned(k) * 0x9e3779b9u;
                             fv = f32_sr(fvd, d == 0 ? rf : rot16(rf));
                         } else {
                             fv = S(fvd);
@@ -921,6 +925,9 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
         // and LDS reads overlap the first cell's arithmetic (FP64 and 3D:
         // slower)
         constexpr int CUNR = sizeof(S) == 4 && D == 2 ? 2 : 1;
+        // FP32: random bits of the stored state's rounding, one hashed word per
+        // lane and tile, stepped by a Weyl increment per owned cell
+        const unsigned rs = mix32(seed ^ (unsigned(t) << 20) ^ unsigned(b * BLOCKP + int(threadIdx.x)) ^ 0x2545f491u);
 #pragma unroll CUNR
         for (int s = slot, j = 0; s < nown; s += NSLOT, ++j) {
             const Item *it = items + s;
@@ -1052,7 +1059,7 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
                     }
                 } else if constexpr (sizeof(S) == 4) {
                     // FP32: the stored deviation stochastically rounded
-                    const unsigned rg = mix32(unsigned(o) ^ seed);
+                    const unsigned rg = rs + unsigned(j) * 0x9e3779b9u;
                     fout_g[o] = f32_sr(og, rg);
                     if (SH) fout_h[o] = f32_sr(oh, rot16(rg));
                 } else {

@@ -240,7 +240,12 @@ def test_mass_conserved_in_closed_cavity(case_factory):
 def test_moment_patch_kinds_agree(case_factory, monkeypatch, name, storage):
     """Face-based and cell-based moment patches (MESO_HIP_MOMENT_CELLS) give
     the same step: the same f-bar+ values, summed in another order (FP64
-    moments), so the macros agree to 1e-12 (FP64 storage) and 1e-9 (FP32)."""
+    moments), so the macros agree to 1e-12 (FP64 storage). FP32 storage rounds
+    f-bar+ stochastically with random bits drawn per lane and tile, which the
+    two patch layouts assign differently: there the two runs differ by that
+    rounding noise, and they are held to the FP32 bound against the oracle
+    (relative L2 <= 1e-6; measured 1e-8 to 1e-7 after 10 steps; a dropped
+    moment term shows at 1e-3)."""
     cs = capi.Case(case_factory(name))
     _say(f"[gpu] moment patch kinds {name} NC={cs.n_cells} NV={cs.n_vel} storage={storage}")
     out = []
@@ -251,7 +256,9 @@ def test_moment_patch_kinds_agree(case_factory, monkeypatch, name, storage):
         dev.step(10)
         out.append(dev.macros())
         dev.close()
-    tol = 1e-12 if storage == capi.STORAGE_FP64 else 1e-9
     for k in keys_of(cs):
-        err = max_rel(out[0][k], out[1][k])
+        if storage == capi.STORAGE_FP64:
+            err, tol = max_rel(out[0][k], out[1][k]), 1e-12
+        else:
+            err, tol = rel_l2(out[0][k], out[1][k]), FP32_TOL
         assert err < tol, f"{name} {k}: {err:.2e}"
