@@ -791,6 +791,93 @@ template <int MODEL, int D, class T, bool TAB>
 __global__ __launch_bounds__(BLOCK) void k_face_eq(KArgs a) {
     d_face_eq<MODEL, D, T, TAB>(a, xcd_block(blockIdx.x, gridDim.x));
 }
+
+// (2') The same stage for Shakhov with a tensor-product velocity set whose
+// weights factorise (the flux moments from 1D central moments): LP lanes per
+// face, BLOCK / LP faces per block. The work of a face that does not scale with
+// the set (its macros, tau, the equilibrium constants, the closed combination
+// of the 1D moments) is the same for 101^2 as for 16^2 velocities; with one
+// face per wave it ran on one lane or on 64 lanes alike, and the axis-table
+// loops left most lanes idle for short axes.
+template <int D, class T, int LP>
+__device__ __forceinline__ void d_face_eq_sep(const KArgs &a, const int b) {
+    if (*a.halt) return;
+    constexpr int MODEL = CDUGKS_SHAKHOV;
+    constexpr int NCOL = ncol_face<MODEL, D>();
+    constexpr int NFL = D + 2;
+    constexpr int FPB = BLOCK / LP;
+    const int f = b * FPB + int(threadIdx.x) / LP;
+    if (f >= a.s.nf) return;  // whole segments leave together
+    const int sl = int(threadIdx.x) % LP;
+    double M[NCOL];
+#pragma unroll
+    for (int k = 0; k < NCOL; ++k) M[k] = 0.0;
+    const double *src = a.part_f + size_t(f) * a.s.ntile_f * NCOL;
+    for (int t = 0; t < a.s.ntile_f; ++t)
+#pragma unroll
+        for (int k = 0; k < NCOL; ++k) M[k] += src[t * NCOL + k];
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int k = 0; k < NCOL; ++k) M[k] += a.mref[k];
+    }
+    const double n[3] = {a.face_n[3 * f], a.face_n[3 * f + 1], a.face_n[3 * f + 2]};
+    EqParams e;
+    double Fb[NFL];
+#pragma unroll
+    for (int k = 0; k < NFL; ++k) Fb[k] = 0.0;
+    face_macros<MODEL, D>(a, M, n, e, Fb);
+    if constexpr (sizeof(T) == 4) relate_shakhov(e, a.ref);
+    if (sl == 0) {
+        a.face_eq[f] = e;
+        if constexpr (sizeof(T) == 4) a.face_eqf[f] = to_eqf(e);
+    }
+    T *gt = static_cast<T *>(a.face_tab) + size_t(f) * a.nt;
+    for (int t = sl; t < a.nt; t += LP) gt[t] = tab_entry<T>(a, e, t);
+    if (a.face_bnd[f] >= 0) return;  // k_boundary forms a boundary face's flux moments
+    double C[3][7];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int k = 0; k < 7; ++k) C[d][k] = 0.0;
+    for (int t = sl; t < a.nt; t += LP) {
+        const int ax = a.tab_ax[t];
+        const double c = a.tab_x[t] - (ax == 0 ? e.u[0] : (ax == 1 ? e.u[1] : e.u[2]));
+        double m = a.tab_w[t] * exp(-e.B * c * c);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                if (d == ax) C[d][k] += m;
+            m *= c;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+#pragma unroll
+        for (int k = 0; k < 7; ++k) C[d][k] = seg_sum<LP>(C[d][k]);
+    if (D == 2) {
+        C[2][0] = 1.0;  // the third axis is the single abscissa 0 with weight 1
+#pragma unroll
+        for (int k = 1; k < 7; ++k) C[2][k] = 0.0;
+    }
+    if (sl == 0) {
+        double full[NFL];
+        shakhov_flux_sep<D>(e, C, n, a.s.K, full);
+        const double mix = e.mix;
+        for (int k = 0; k < NFL; ++k) a.face_flux[size_t(f) * NFL + k] = (1.0 - mix) * Fb[k] + mix * full[k];
+    }
+}
+template <int D, class T, int LP>
+__global__ __launch_bounds__(BLOCK) void k_face_eq_sep(KArgs a) {
+    d_face_eq_sep<D, T, LP>(a, xcd_block(blockIdx.x, gridDim.x));
+}
+template <int D, class T, int LP>
+__global__ __launch_bounds__(BLOCK) void kb_face_eq_sep(const KArgs *__restrict__ tab, const int2 *__restrict__ map) {
+    const int2 m = map[blockIdx.x];
+    if (m.x < 0) return;  // padding of a short XCD list
+    d_face_eq_sep<D, T, LP>(batch_args(tab, m.x), m.y);
+}
+constexpr int FEQ_LP = 16;  // lanes per face of k_face_eq_sep
 template <int MODEL, int D, class T, bool TAB>
 __global__ __launch_bounds__(BLOCK) void kb_face_eq(const KArgs *__restrict__ tab, const int2 *__restrict__ map) {
     const int2 m = map[blockIdx.x];
@@ -2194,6 +2281,10 @@ struct Launch {
         hipLaunchKernelGGL(kern, dim3(grid), dim3(BLOCKP), shmem, (ctx)->stream, __VA_ARGS__);       \
     } while (0)
 
+// Whether the face stage is k_face_eq_sep: Shakhov with axis tables and
+// factorised weights.
+bool face_sep(const meso_hip_ctx *c) { return c->s.model == CDUGKS_SHAKHOV && c->nt > 0 && c->tab_w != nullptr; }
+
 // Step with the LDS-staged patch kernels for the face moments and the
 // interior cells; boundary faces and mirror-boundary cells keep their kernels.
 template <int MODEL, class T, bool TAB, int D, bool LIM, bool X>
@@ -2221,7 +2312,10 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
         LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, false, X>), c->n_mp * c->p_chunks,
                  c->lds_mp, a);
     }
-    LAUNCH_SH(c, "face_eq", (k_face_eq<MODEL, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
+    if (MODEL == CDUGKS_SHAKHOV && TAB && face_sep(c))
+        LAUNCH(c, "face_eq", (k_face_eq_sep<D, T, FEQ_LP>), (s.nf + BLOCK / FEQ_LP - 1) / (BLOCK / FEQ_LP), a);
+    else
+        LAUNCH_SH(c, "face_eq", (k_face_eq<MODEL, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
     if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MODEL, D, T, LIM>), s.nb, 0, a);
     if (c->n_int > 0) {
         LAUNCH(c, "cell_prep", (k_cell_prep<MODEL, T, TAB, D>), (c->n_int + WPB - 1) / WPB, a);
@@ -2259,7 +2353,10 @@ int step_body(meso_hip_ctx *c, KArgs a) {
     }
     const int fwblk = (s.nf + WPB - 1) / WPB;
     LAUNCH_SH(c, "face_moments", (k_face_moments<MF, D, T, LIM>), fwblk * s.ntile_f, 0, a);
-    LAUNCH_SH(c, "face_eq", (k_face_eq<MF, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
+    if (MF == CDUGKS_SHAKHOV && TAB && face_sep(c))
+        LAUNCH(c, "face_eq", (k_face_eq_sep<D, T, FEQ_LP>), (s.nf + BLOCK / FEQ_LP - 1) / (BLOCK / FEQ_LP), a);
+    else
+        LAUNCH_SH(c, "face_eq", (k_face_eq<MF, D, T, TAB>), fwblk, TAB ? size_t(WPB) * c->nt * sizeof(T) : 0, a);
     if (s.nb > 0) LAUNCH_SH(c, "boundary", (k_boundary<MF, D, T, LIM>), s.nb, 0, a);
     const size_t lds = TAB ? size_t(WPB) * (MAXF + 2) * c->nt * sizeof(T) : 0;
     if (c->n_int > 0)
@@ -3750,8 +3847,8 @@ struct BatchGroup {
 // Key of the kernel instance a context's step launches (contexts that share
 // it share the launches).
 long batch_key(const meso_hip_ctx *c) {
-    return ((((long(c->s.model) * 4 + c->s.D) * 2 + (c->storage == MESO_STORAGE_FP32)) * 2 + c->ext) * 2 +
-            (c->nt > 0)) * 4 + c->s.limiter * 2 + c->mp_cells;
+    return ((((long(c->s.model) * 4 + c->s.D) * 2 + (c->storage == MESO_STORAGE_FP32)) * 2 + c->ext) * 3 +
+            (c->nt > 0 ? (face_sep(c) ? 2 : 1) : 0)) * 4 + c->s.limiter * 2 + c->mp_cells;
 }
 
 // A context can join the fused launches when its whole step is the patch
@@ -3961,7 +4058,8 @@ int batch_build(meso_hip_batch *b) {
             meso_hip_ctx *c = b->ctx[g.members[i]];
             const StepConst &s = c->s;
             const double dvc = double(s.nc) * s.nv;
-            const int cnt[BS_N] = {c->n_mp * c->p_chunks, (s.nf + WPB - 1) / WPB, s.nb, (c->n_int + WPB - 1) / WPB,
+            const int fpb = face_sep(c) ? BLOCK / FEQ_LP : WPB;  // faces per block of the face stage
+            const int cnt[BS_N] = {c->n_mp * c->p_chunks, (s.nf + fpb - 1) / fpb, s.nb, (c->n_int + WPB - 1) / WPB,
                                    c->n_up * c->p_chunks,
                                    s.model == CDUGKS_SHAKHOV ? (s.nc + BLOCK - 1) / BLOCK : 0};
             const double work[BS_N] = {dvc, double(s.nf) * std::max(c->nt, 1), double(s.nb) * s.nv,
@@ -4060,7 +4158,10 @@ int batch_group_step(meso_hip_batch *b, const BatchGroup &g, int phase, bool cel
         if ((rc = batch_lds_attr(b, km, g.lds_mp))) return rc;
         BLAUNCH(st, km, g.blocks[BS_MOM], BLOCKP, g.lds_mp, tab, g.map[BS_MOM]);
     }
-    BLAUNCH(st, (kb_face_eq<MODEL, D, T, TAB>), g.blocks[BS_FEQ], BLOCK, TAB ? g.lds_feq : 0, tab, g.map[BS_FEQ]);
+    if (MODEL == CDUGKS_SHAKHOV && TAB && face_sep(b->ctx[g.members[0]]))
+        BLAUNCH(st, (kb_face_eq_sep<D, T, FEQ_LP>), g.blocks[BS_FEQ], BLOCK, 0, tab, g.map[BS_FEQ]);
+    else
+        BLAUNCH(st, (kb_face_eq<MODEL, D, T, TAB>), g.blocks[BS_FEQ], BLOCK, TAB ? g.lds_feq : 0, tab, g.map[BS_FEQ]);
     if (g.blocks[BS_BND]) BLAUNCH(st, (kb_boundary<MODEL, D, T, LIM>), g.blocks[BS_BND], BLOCK, 0, tab, g.map[BS_BND]);
     BLAUNCH(st, (kb_cell_prep<MODEL, T, TAB, D>), g.blocks[BS_PREP], BLOCK, 0, tab, g.map[BS_PREP]);
     auto *ku = &kb_patch_update<MODEL, T, VTU, TAB, D, LIM, X>;
