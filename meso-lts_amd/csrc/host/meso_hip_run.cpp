@@ -4,8 +4,11 @@
 // (include/solver/solver.h:22-46): initial(), do_step(), output(),
 // get_run_state(), max_step(), write_interval(); main() is handle_solver's loop.
 //
-// usage: meso_hip_run --case <case-file> [--storage fp64|fp32] [--out <dir>] [--not-run]
+// usage: meso_hip_run --case <case-file> [--storage fp64|fp32|fp32x] [--out <dir>] [--not-run]
 //   --out: directory under which <case-name>/ is written (meso: the working directory)
+//   --storage: FP64 (meso's precision), FP32 (4 bytes per value, stochastically
+//     rounded: its residual settles near 1e-6, so a residual-limit below that
+//     is not reached) or FP32X (5 bytes per value, converges with FP64)
 #include "meso_hip.h"
 
 #include <cmath>
@@ -229,18 +232,21 @@ int main(int argc, char **argv) {
         else if (a == "--out" && i + 1 < argc) out = argv[++i];
         else if (a == "--not-run") not_run = true;
         else {
-            std::cerr << "usage: meso_hip_run --case <case-file> [--storage fp64|fp32] [--out <dir>] [--not-run]"
+            std::cerr << "usage: meso_hip_run --case <case-file> [--storage fp64|fp32|fp32x] [--out <dir>] [--not-run]"
                       << std::endl;
             return 2;
         }
     }
-    if (case_path.empty() || (storage_s != "fp64" && storage_s != "fp32")) {
-        std::cerr << "usage: meso_hip_run --case <case-file> [--storage fp64|fp32] [--out <dir>] [--not-run]"
+    if (case_path.empty() || (storage_s != "fp64" && storage_s != "fp32" && storage_s != "fp32x")) {
+        std::cerr << "usage: meso_hip_run --case <case-file> [--storage fp64|fp32|fp32x] [--out <dir>] [--not-run]"
                   << std::endl;
         return 2;
     }
     try {
-        HipSolver solver(case_path, storage_s == "fp32" ? MESO_STORAGE_FP32 : MESO_STORAGE_FP64, out);
+        const int storage = storage_s == "fp32"    ? MESO_STORAGE_FP32
+                            : storage_s == "fp32x" ? MESO_STORAGE_FP32X
+                                                   : MESO_STORAGE_FP64;
+        HipSolver solver(case_path, storage, out);
         solver.initial();
         solver.output();
         if (not_run) return 0;
