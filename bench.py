@@ -10,12 +10,16 @@ synchronisation on both sides.
 The step runs in three storages of the distributions g, h:
   fp32   FP32 words (deviation from the rest Maxwellian, stochastically
          rounded), FP64 transport arithmetic, FP32 equilibria
-  fp32x  an FP32 word plus a byte extending its significand to 31 bits (5 B),
-         FP64 arithmetic
+  fp32e  an FP32 word plus a byte extending its significand to 31 bits (5 B),
+         the FP32 kernels
+  fp32x  the same format, the FP64 kernels
   fp64   meso's own precision
 The line's value is the fastest storage whose every macro field is within
-1e-6 relative L2 of the FP64 run at meso's convergence (at_convergence); with
-no convergence leg, FP32 (--headline picks one).
+1e-6 relative L2 of the FP64 run at meso's convergence, in this run's
+reduced-resolution leg (at_convergence) and in the committed full-resolution
+measurement (profiles/convergence_<workload>_full.json,
+at_convergence_full_resolution); with no convergence leg, FP32 (--headline
+picks one).
 Reported per storage: ms/step, DV-cell updates/s, the roofline of the
 dominant kernel (algorithmic bytes: read + write of every stored g, h value of
 the DV-cells the launch owns, over its average launch time from HIP events),
@@ -26,7 +30,7 @@ FP64 oracle with OpenMP on this host (one step of the same workload) and its
 measured speed ratio to meso itself.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--scale s]
-                       [--storage all|fp32x|fp32|fp64] [--headline auto|fp32x|fp32|fp64]
+                       [--storage all|fp32x|fp32e|fp32|fp64] [--headline auto|fp32x|fp32e|fp32|fp64]
                        [--converge-scale s] [--copy-calibration MIB]
                        [--autotune] [--graphs] [--batch N]
        python bench.py --converge [--workload W] [--scale s] [--max-steps N] [--time-limit S] [--run-on]
@@ -240,7 +244,8 @@ def converge_run(workload, scale, max_steps, time_limit, residual_limit=0.0, cur
     if residual_limit > 0:
         limit = residual_limit
     idx = ([0, 4, 1, 2, 5, 6] if dim == 2 else [0, 4, 1, 2, 3, 5, 6, 7]) if shakhov else list(range(1 + dim))
-    kinds = [("fp64", capi.STORAGE_FP64), ("fp32", capi.STORAGE_FP32), ("fp32x", capi.STORAGE_FP32X)]
+    kinds = [("fp64", capi.STORAGE_FP64), ("fp32", capi.STORAGE_FP32), ("fp32e", capi.STORAGE_FP32E),
+             ("fp32x", capi.STORAGE_FP32X)]
     devs = {}
     for tag, st in kinds:
         try:
@@ -328,9 +333,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--scale", type=float, default=1.0, help="mesh resolution scale (<1: quick runs)")
     ap.add_argument("--workload", default=WORKLOAD)
-    ap.add_argument("--storage", choices=("all", "fp32x", "fp32", "fp64"), default="all",
+    ap.add_argument("--storage", choices=("all", "fp32x", "fp32e", "fp32", "fp64"), default="all",
                     help="storage kinds to run (profiling runs use one; the JSON line needs all)")
-    ap.add_argument("--headline", choices=("auto", "fp32x", "fp32", "fp64"), default="auto",
+    ap.add_argument("--headline", choices=("auto", "fp32x", "fp32e", "fp32", "fp64"), default="auto",
                     help="storage whose rate is the line's value (auto: the fastest one within 1e-6 of FP64 "
                          "on every macro field at convergence)")
     ap.add_argument("--converge-scale", type=float, default=0.25,
@@ -387,7 +392,8 @@ def main():
 
     res = {}
     macros = {}
-    kinds = [(capi.STORAGE_FP32X, "fp32x"), (capi.STORAGE_FP32, "fp32"), (capi.STORAGE_FP64, "fp64")]
+    kinds = [(capi.STORAGE_FP32X, "fp32x"), (capi.STORAGE_FP32E, "fp32e"), (capi.STORAGE_FP32, "fp32"),
+             (capi.STORAGE_FP64, "fp64")]
     if args.storage != "all":
         kinds = [k for k in kinds if k[1] == args.storage]
     tiles = {}
@@ -414,7 +420,7 @@ def main():
         return
     dim = cs.mesh.dim
     # bytes per stored distribution value: FP32X keeps an FP32 word and a byte
-    sbytes = {"fp32": 4, "fp32x": 5, "fp64": 8}
+    sbytes = {"fp32": 4, "fp32x": 5, "fp32e": 5, "fp64": 8}
 
     def roofline(tag):
         """Dominant kernel of a storage: algorithmic bytes (read + write of
@@ -464,15 +470,29 @@ def main():
         except Exception as e:  # the baseline is reported, never required
             log(f"[bench] cpu baseline failed: {e}")
 
+    # the full-resolution run of the workload to meso's convergence (about 25
+    # minutes of GPU time for the four storages, so a committed measurement,
+    # not part of this run)
+    full = None
+    fpath = os.path.join(ROOT, "profiles", f"convergence_{args.workload}_full.json")
+    if args.scale == 1.0 and os.path.exists(fpath):
+        with open(fpath) as fp:
+            full = json.load(fp)
     head = args.headline
     if head == "auto":
-        # the fastest storage that holds the precision target at convergence
+        # the fastest storage that holds the precision target at convergence:
+        # in this run's reduced-resolution leg and in the committed
+        # full-resolution measurement
         head = "fp32"
         if conv and conv.get("at_fp64_convergence"):
             errs = conv["at_fp64_convergence"]["errors"]
-            ok = [t for t in ("fp32", "fp32x") if t in errs and max(errs[t].values()) <= 1e-6] + ["fp64"]
-            head = max(ok, key=lambda t: res[t]["ups"])
+            ok = [t for t in ("fp32", "fp32e", "fp32x") if t in errs and t in res and max(errs[t].values()) <= 1e-6]
+            if full:
+                ok = [t for t in ok if t not in full["errors"] or max(full["errors"][t].values()) <= 1e-6]
+            head = max(ok + ["fp64"], key=lambda t: res[t]["ups"])
     storage_txt = {"fp32x": "fp32 word + 8-bit significand extension per g,h value (5 B); fp64 arithmetic",
+                   "fp32e": "fp32 word + 8-bit significand extension per g,h value (5 B); the fp32 kernels (fp32 "
+                            "face values and fluxes, stochastically rounded; fp64 transport arithmetic and macros)",
                    "fp32": "fp32 g,h (deviation from the rest Maxwellian, stochastically rounded); fp64 transport "
                            "arithmetic and moments/macros",
                    "fp64": "fp64"}
@@ -497,6 +517,7 @@ def main():
         "step_hbm": per[head]["step_hbm"],
         "storages": per,
         "at_convergence": conv,
+        "at_convergence_full_resolution": full,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

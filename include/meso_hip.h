@@ -44,9 +44,14 @@ extern "C" {
 
 #define MESO_STORAGE_FP64 0 /* f, fluxes, gradients in FP64 */
 #define MESO_STORAGE_FP32 1 /* f, fluxes, gradients in FP32; moments, macros, residual in FP64 */
-/* f as FP32 plus an 8-bit extension of its mantissa (a 32-bit significand,
- * 5 bytes per value); face values, moments and macros as for FP32 */
+/* f as FP32 plus an 8-bit extension of its mantissa (a 31-bit significand of
+ * the deviation from the rest Maxwellian, 5 bytes per value); the step runs
+ * the FP64 kernels (face values, fluxes and equilibria in FP64) */
 #define MESO_STORAGE_FP32X 2
+/* f in the FP32X format (31-bit significand, 5 bytes per value); the step runs
+ * the FP32 kernels (face values and fluxes handed over in FP32, stochastically
+ * rounded; equilibria in FP32; transport arithmetic, moments and macros in FP64) */
+#define MESO_STORAGE_FP32E 3
 
 #define MESO_LAYOUT_VELOCITY_MAJOR 0
 #define MESO_LAYOUT_CELL_MAJOR 1

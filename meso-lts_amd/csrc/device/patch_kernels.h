@@ -170,7 +170,7 @@ template <int ND, int VT>
 __device__ __forceinline__ void stage_values_x(const KArgs &a, const int *rows, int nr, int nvp, int v, bool ok,
                                                const double ref[2], int slot, int vl, double *val, int maxr) {
     constexpr int NSLOT = (BLOCKP / 64) * (64 / VT);
-    constexpr int B = 8;
+    constexpr int B = 4;  // rows per batch (8 raised the moment kernel past the registers of 7 waves per SIMD: 8.3 ms against 6.5)
     const float *wg = (const float *)a.fin_g, *wh = (const float *)a.fin_h;
     const unsigned *xg = a.xin_g.as<const unsigned>(), *xh = a.xin_h.as<const unsigned>();
     for (int base = slot; base < nr; base += NSLOT * B) {
@@ -197,6 +197,50 @@ __device__ __forceinline__ void stage_values_x(const KArgs &a, const int *rows, 
             if (r < nr)
 #pragma unroll
                 for (int d = 0; d < ND; ++d) val[(d * maxr + r) * VT + vl] = ok ? x32_decode(w[k][d], e[k][d]) + ref[d] : 0.0;
+        }
+    }
+}
+
+// FP32E storage (FP32 kernels on the FP32X format): the rows hold the FP32
+// word and extension byte of the deviation from f_ref; they are staged as FP32
+// values stochastically rounded from the 31-bit deviation, so the gradients
+// and face values see the state with an unbiased FP32 rounding each step while
+// the state itself keeps 31 bits. The word is the value truncated toward zero
+// and the byte the next 8 bits of its significand, so the stochastic rounding
+// is the word moved one unit away from zero with probability byte / 256 (a
+// random byte below it): integer operations only (r: one hashed word per lane
+// and tile, stepped per row).
+template <int ND, int VT>
+__device__ __forceinline__ void stage_values_xf(const KArgs &a, const int *rows, int nr, int nvp, int v, bool ok,
+                                                int slot, int vl, float *val, int maxr, unsigned r) {
+    constexpr int NSLOT = (BLOCKP / 64) * (64 / VT);
+    constexpr int B = 4;  // rows per batch (8 raised the moment kernel past the registers of 7 waves per SIMD: 8.3 ms against 6.5)
+    const float *wg = (const float *)a.fin_g, *wh = (const float *)a.fin_h;
+    const unsigned *xg = a.xin_g.as<const unsigned>(), *xh = a.xin_h.as<const unsigned>();
+    for (int base = slot; base < nr; base += NSLOT * B) {
+        float w[B][ND];
+        unsigned e[B][ND];
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int rr = base + k * NSLOT;
+            const bool on = ok && rr < nr;
+            const size_t o = size_t(on ? rows[rr] : 0) * nvp + (on ? v : 0);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                w[k][d] = on ? (d == 0 ? wg : wh)[o] : 0.0f;
+                e[k][d] = on ? (d == 0 ? xg : xh)[o >> 2] >> (8 * (o & 3)) : 0u;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < B; ++k) {
+            const int rr = base + k * NSLOT;
+            const unsigned rk = r + unsigned(base + k * NSLOT) * 0x9e3779b9u;
+            if (rr < nr)
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const unsigned up = ((rk >> (8 * d)) & 0xffu) < (e[k][d] & 0xffu) ? 1u : 0u;
+                    val[(d * maxr + rr) * VT + vl] = ok ? __uint_as_float(__float_as_uint(w[k][d]) + up) : 0.0f;
+                }
         }
     }
 }
@@ -229,7 +273,7 @@ template <int ND, class S, int VT>
 __device__ __forceinline__ void stage_values(const int *rows, int nr, const S *fin_g, const S *fin_h, int nvp, int v,
                                              bool ok, int slot, int vl, S *val, int maxr) {
     constexpr int NSLOT = (BLOCKP / 64) * (64 / VT);
-    constexpr int B = 8;
+    constexpr int B = 4;  // rows per batch (8 raised the moment kernel past the registers of 7 waves per SIMD: 8.3 ms against 6.5)
     for (int base = slot; base < nr; base += NSLOT * B) {
         S x[B][ND];
 #pragma unroll
@@ -510,10 +554,13 @@ __device__ __forceinline__ void d_patch_moments(const KArgs &a, const int b) {
         const double4 X = a.vel_d[ok ? v : 0];
         // A: stage values and tile velocities, clear the face rows
         double ref[2] = {0.0, 0.0};  // XS: f_ref of the tile velocity (g, h)
-        if constexpr (XS) {
+        if constexpr (XS && sizeof(S) == 8) {
             ref[0] = ok ? a.fref_d[v] : 0.0;
             ref[1] = ok && SH ? a.fref_d[a.s.nv + v] : 0.0;
             stage_values_x<ND, VT>(a, rows, nr, nvp, v, ok, ref, slot, vl, (double *)val, maxr);
+        } else if constexpr (XS) {
+            stage_values_xf<ND, VT>(a, rows, nr, nvp, v, ok, slot, vl, (float *)val, maxr,
+                                    mix32(fseed ^ 0x3c6ef372u ^ (unsigned(t) << 20) ^ unsigned(b * BLOCKP + int(threadIdx.x))));
         } else {
             stage_values<ND, S, VT>(rows, nr, fin_g, fin_h, nvp, v, ok, slot, vl, val, maxr);
         }
@@ -834,7 +881,7 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
         const double xv[3] = {L.x, L.y, L.z};
         // A: stage the owned cells' values (read in C only: no barrier before B)
         double ref[2] = {0.0, 0.0};  // XS: f_ref of the lane velocity (g, h)
-        if constexpr (XS) {
+        if constexpr (XS && sizeof(S) == 8) {
             ref[0] = L.ok ? a.fref_d[v] : 0.0;
             ref[1] = L.ok && SH ? a.fref_d[a.s.nv + v] : 0.0;
             stage_values_x<ND, VT>(a, rows, nown, nvp, v, L.ok, ref, slot, vl, (double *)val, maxr);
@@ -961,8 +1008,17 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
             }
             const CellRec<S, D> &R = crl[s];
             double qv[3] = {0.0, 0.0, 0.0};
-            const double fcg = double(val[s * VT + vl]);
-            const double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
+            double fcg = double(val[s * VT + vl]);
+            double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
+            if constexpr (XS && sizeof(S) == 4) {
+                // FP32E: the staged word and its extension byte (the dword
+                // that holds it, shared by four lanes)
+                const size_t o = size_t(c) * nvp + (L.ok ? v : 0);
+                fcg = x32_decode(val[s * VT + vl], a.xin_g.template as<const unsigned>()[o >> 2] >> (8 * (o & 3)));
+                if (SH)
+                    fch = x32_decode(val[(maxr + s) * VT + vl],
+                                     a.xin_h.template as<const unsigned>()[o >> 2] >> (8 * (o & 3)));
+            }
             double og, oh = 0.0;
             if (SH) {
                 double gsn, hsn, gs, hs;
@@ -1043,7 +1099,21 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
             }
             if (L.ok) {
                 const size_t o = size_t(c) * nvp + v;
-                if constexpr (XS) {
+                if constexpr (XS && sizeof(S) == 4) {
+                    // FP32E: the deviation (this kernel's arithmetic form),
+                    // stochastically rounded at bit 31
+                    float wd;
+                    unsigned char e;
+                    const unsigned rg = rs + unsigned(j) * 0x9e3779b9u;
+                    x32_encode_sr(og, rg, wd, e);
+                    fout_g[o] = wd;
+                    a.xout_g[o] = e;
+                    if (SH) {
+                        x32_encode_sr(oh, rot16(rg), wd, e);
+                        fout_h[o] = wd;
+                        a.xout_h[o] = e;
+                    }
+                } else if constexpr (XS) {
                     // deviation from f_ref, stochastically rounded at bit 31
                     // (random bits from the entry and the step)
                     float wd;

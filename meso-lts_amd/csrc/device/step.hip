@@ -2345,9 +2345,8 @@ int step_body(meso_hip_ctx *c, KArgs a) {
     constexpr int MAXF = D == 2 ? 4 : 6;
     if constexpr (MODEL != DUGKS_BGK) {
         if (c->patches) {
-            // FP32X: the FP64 kernels on FP32 words + extension bytes
-            if constexpr (sizeof(T) == 8)
-                if (c->ext) return step_patch<MODEL, T, TAB, D, LIM, true>(c, a);
+            // FP32X / FP32E: the FP64 / FP32 kernels on FP32 words + extension bytes
+            if (c->ext) return step_patch<MODEL, T, TAB, D, LIM, true>(c, a);
             return step_patch<MODEL, T, TAB, D, LIM, false>(c, a);
         }
     }
@@ -2705,12 +2704,15 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
         return set_err(nullptr, MESO_E_INVALID, "unknown model");
     if (m->dim != 2 && m->dim != 3) return set_err(nullptr, MESO_E_INVALID, "dimension must be 2 or 3");
     if (p->dim != m->dim) return set_err(nullptr, MESO_E_INVALID, "params.dim differs from the mesh dimension");
-    if (storage != MESO_STORAGE_FP64 && storage != MESO_STORAGE_FP32 && storage != MESO_STORAGE_FP32X)
+    if (storage != MESO_STORAGE_FP64 && storage != MESO_STORAGE_FP32 && storage != MESO_STORAGE_FP32X &&
+        storage != MESO_STORAGE_FP32E)
         return set_err(nullptr, MESO_E_INVALID,
-                       "storage must be MESO_STORAGE_FP64, MESO_STORAGE_FP32 or MESO_STORAGE_FP32X");
-    // FP32X runs the FP64 kernels on words + extension bytes in HBM
-    const bool ext = storage == MESO_STORAGE_FP32X;
-    if (ext) storage = MESO_STORAGE_FP64;
+                       "storage must be MESO_STORAGE_FP64, MESO_STORAGE_FP32, MESO_STORAGE_FP32X or MESO_STORAGE_FP32E");
+    // FP32X runs the FP64 kernels, FP32E the FP32 kernels, on words +
+    // extension bytes in HBM
+    const bool ext = storage == MESO_STORAGE_FP32X || storage == MESO_STORAGE_FP32E;
+    if (storage == MESO_STORAGE_FP32X) storage = MESO_STORAGE_FP64;
+    if (storage == MESO_STORAGE_FP32E) storage = MESO_STORAGE_FP32;
     if (m->n_cells <= 0 || m->n_faces <= 0 || d->n_vel <= 0) return set_err(nullptr, MESO_E_INVALID, "empty mesh or DVS");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -3213,7 +3215,7 @@ int meso_hip_create(const meso_mesh_desc *m, const meso_dvs_desc *d, const meso_
     c->w_h = w_s;
     if (ext && (!c->patches || c->n_bcell > 0 || p->model == MESO_MODEL_DUGKS_BGK))
         return fail(set_err(nullptr, MESO_E_INVALID,
-                            "MESO_STORAGE_FP32X needs the patch kernels: a CDUGKS model, at least 64 velocities "
+                            "MESO_STORAGE_FP32X and FP32E need the patch kernels: a CDUGKS model, at least 64 velocities "
                             "and no symmetry or slip-wall boundary"));
     if (hipHostMalloc((void **)&c->flags_host, FLAG_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess)
         return fail(set_err(nullptr, MESO_E_DEVICE, "hipHostMalloc failed"));
@@ -4181,7 +4183,9 @@ int batch_group_dispatch(meso_hip_batch *b, const BatchGroup &g, int phase) {
                x = c->ext, cells = c->mp_cells;
 #define MESO_BL(M, DD, TB, L)                                                                                    \
     if (m == M && D == DD && tab == TB && lim == L) {                                                            \
-        if (f32) return batch_group_step<M, float, TB, DD, L, false>(b, g, phase, cells);                       \
+        if (f32)                                                                                                 \
+            return x ? batch_group_step<M, float, TB, DD, L, true>(b, g, phase, cells)                          \
+                     : batch_group_step<M, float, TB, DD, L, false>(b, g, phase, cells);                        \
         return x ? batch_group_step<M, double, TB, DD, L, true>(b, g, phase, cells)                             \
                  : batch_group_step<M, double, TB, DD, L, false>(b, g, phase, cells);                           \
     }

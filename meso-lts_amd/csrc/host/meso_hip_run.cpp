@@ -4,11 +4,12 @@
 // (include/solver/solver.h:22-46): initial(), do_step(), output(),
 // get_run_state(), max_step(), write_interval(); main() is handle_solver's loop.
 //
-// usage: meso_hip_run --case <case-file> [--storage fp64|fp32|fp32x] [--out <dir>] [--not-run]
+// usage: meso_hip_run --case <case-file> [--storage fp64|fp32|fp32x|fp32e] [--out <dir>] [--not-run]
 //   --out: directory under which <case-name>/ is written (meso: the working directory)
 //   --storage: FP64 (meso's precision), FP32 (4 bytes per value, stochastically
 //     rounded: its residual settles near 1e-6, so a residual-limit below that
-//     is not reached) or FP32X (5 bytes per value, converges with FP64)
+//     is not reached), FP32X (5 bytes per value, the FP64 kernels; converges
+//     with FP64) or FP32E (the FP32X format with the FP32 kernels)
 #include "meso_hip.h"
 
 #include <cmath>
@@ -232,19 +233,21 @@ int main(int argc, char **argv) {
         else if (a == "--out" && i + 1 < argc) out = argv[++i];
         else if (a == "--not-run") not_run = true;
         else {
-            std::cerr << "usage: meso_hip_run --case <case-file> [--storage fp64|fp32|fp32x] [--out <dir>] [--not-run]"
+            std::cerr << "usage: meso_hip_run --case <case-file> [--storage fp64|fp32|fp32x|fp32e] [--out <dir>] [--not-run]"
                       << std::endl;
             return 2;
         }
     }
-    if (case_path.empty() || (storage_s != "fp64" && storage_s != "fp32" && storage_s != "fp32x")) {
-        std::cerr << "usage: meso_hip_run --case <case-file> [--storage fp64|fp32|fp32x] [--out <dir>] [--not-run]"
+    if (case_path.empty() || (storage_s != "fp64" && storage_s != "fp32" && storage_s != "fp32x" &&
+                               storage_s != "fp32e")) {
+        std::cerr << "usage: meso_hip_run --case <case-file> [--storage fp64|fp32|fp32x|fp32e] [--out <dir>] [--not-run]"
                   << std::endl;
         return 2;
     }
     try {
         const int storage = storage_s == "fp32"    ? MESO_STORAGE_FP32
                             : storage_s == "fp32x" ? MESO_STORAGE_FP32X
+                            : storage_s == "fp32e" ? MESO_STORAGE_FP32E
                                                    : MESO_STORAGE_FP64;
         HipSolver solver(case_path, storage, out);
         solver.initial();

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libmeso_hip.so")
 MESO_OK = 0
 MESO_E_INVALID, MESO_E_IO, MESO_E_DEVICE, MESO_E_NUMERIC = -1, -2, -3, -4
 MODEL_CDUGKS_BGK, MODEL_CDUGKS_SHAKHOV, MODEL_DUGKS_BGK = 1, 2, 3
-STORAGE_FP64, STORAGE_FP32, STORAGE_FP32X = 0, 1, 2
+STORAGE_FP64, STORAGE_FP32, STORAGE_FP32X, STORAGE_FP32E = 0, 1, 2, 3
 LAYOUT_VELOCITY_MAJOR, LAYOUT_CELL_MAJOR = 0, 1
 BC = dict(fluid_interior=0, farfield_inlet=1, pressure_inlet=2, freestream_inlet=3, farfield_outlet=4,
           pressure_outlet=5, wall=6, slip_wall=7, symmetry=8)

@@ -64,7 +64,7 @@ def run_batch(cases, storage):
     return out
 
 
-@pytest.mark.parametrize("storage", [capi.STORAGE_FP64, capi.STORAGE_FP32X, capi.STORAGE_FP32])
+@pytest.mark.parametrize("storage", [capi.STORAGE_FP64, capi.STORAGE_FP32X, capi.STORAGE_FP32E, capi.STORAGE_FP32])
 def test_batch64_runs_and_conserves_mass(cases, storage):
     dvc = sum(cs.n_cells * cs.n_vel for cs in cases)
     _say(f"[gpu] batch of {len(cases)} storage={storage}: {dvc:.3e} DV-cells")
@@ -74,7 +74,8 @@ def test_batch64_runs_and_conserves_mass(cases, storage):
     _results[storage] = [m for m, _ in res]
 
 
-@pytest.mark.parametrize("storage,tol", [(capi.STORAGE_FP32X, FP32X_TOL), (capi.STORAGE_FP32, FP32_TOL)])
+@pytest.mark.parametrize("storage,tol", [(capi.STORAGE_FP32X, FP32X_TOL), (capi.STORAGE_FP32E, FP32X_TOL),
+                                         (capi.STORAGE_FP32, FP32_TOL)])
 def test_batch64_reduced_storage_against_fp64(cases, storage, tol):
     if capi.STORAGE_FP64 not in _results or storage not in _results:
         pytest.skip("needs the batch runs above")

@@ -59,7 +59,7 @@ def assert_same(alone, fused, what):
 
 
 @pytest.mark.parametrize("graphs", [False, True])
-@pytest.mark.parametrize("storage", [capi.STORAGE_FP32, capi.STORAGE_FP32X, capi.STORAGE_FP64])
+@pytest.mark.parametrize("storage", [capi.STORAGE_FP32, capi.STORAGE_FP32E, capi.STORAGE_FP32X, capi.STORAGE_FP64])
 def test_fused_batch_equals_one_by_one(cases, storage, graphs):
     _say(f"[gpu] fused batch storage={storage} graphs={graphs}:",
          ", ".join(f"{c.n_cells}x{c.n_vel}" for c in cases))

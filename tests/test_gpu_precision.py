@@ -7,7 +7,9 @@ Maxwellian f_ref): the cell state between steps (stochastically rounded at bit
 (rounded to nearest). All arithmetic is FP64, the step of FP64 storage.
 
 Tolerance: relative L2 <= 1e-7 on every macro component after 100-200 steps,
-ten times below the 1e-6 the precision target allows. (On the Kn = 1 cavity at
+ten times below the 1e-6 the precision target allows. FP32E storage (the
+FP32X format run by the FP32 kernels: face values and fluxes FP32,
+stochastically rounded) is held to 1e-6 there. (On the Kn = 1 cavity at
 half resolution, run to meso's convergence rule, FP32X stays at ~4e-9 of the
 FP64 run and converges at the same step; FP32 storage with round-to-nearest
 stagnates at 5e-4.)
@@ -22,6 +24,14 @@ from oracle.oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 FP32X_TOL = 1e-7
+# FP32E (the same format, the FP32 kernels): its state keeps 31 bits, its face
+# values and flux rows are FP32 stochastically rounded; held to the precision
+# target (FP32 storage's own bound)
+FP32E_TOL = 1e-6
+# ... except on the two channels, whose small cross-flow components (v, q)
+# carry the FP32 rounding of the face values at a larger relative size (FP32
+# storage's rounding floor there is 3e-6 after 200 steps)
+FP32E_TOL_CHANNEL = 1e-5
 
 
 def _say(*a):
@@ -47,21 +57,24 @@ FP32X_CASES = {"shak_quad6": 200, "shak_hex3": 200, "shak_quad12_nc24": 200, "sh
                "shak_hex3_lim": 200, "kn0075_cavity": 100}
 
 
+@pytest.mark.parametrize("storage,tol", [(capi.STORAGE_FP32X, FP32X_TOL), (capi.STORAGE_FP32E, FP32E_TOL)])
 @pytest.mark.parametrize("name", list(FP32X_CASES))
-def test_fp32x_storage_close_to_oracle(case_factory, name):
+def test_fp32x_storage_close_to_oracle(case_factory, name, storage, tol):
     cs = capi.Case(case_factory(name))
     steps = FP32X_CASES[name]
-    _say(f"[gpu] fp32x {name} NC={cs.n_cells} NV={cs.n_vel} steps={steps}")
-    dev = capi.Device(cs, capi.STORAGE_FP32X)
+    _say(f"[gpu] storage {storage} {name} NC={cs.n_cells} NV={cs.n_vel} steps={steps}")
+    dev = capi.Device(cs, storage)
     orc = Oracle(cs)
     dev.init_equilibrium()
     orc.init_equilibrium()
     dev.step(steps)
     orc.step(steps)
     err = macro_errors(cs, dev.macros(), orc.macros())
-    _say(f"[gpu] fp32x {name}: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()))
+    _say(f"[gpu] storage {storage} {name}: " + " ".join(f"{k}={v:.2e}" for k, v in err.items()))
+    if storage == capi.STORAGE_FP32E and name.startswith("shak_channel"):
+        tol = FP32E_TOL_CHANNEL
     for k, v in err.items():
-        assert v < FP32X_TOL, f"{name} {k}: {v:.2e}"
+        assert v < tol, f"{name} {k}: {v:.2e}"
 
 
 def test_fp32x_state_roundtrip(case_factory):
@@ -88,11 +101,12 @@ def test_fp32x_state_roundtrip(case_factory):
         assert v < FP32X_TOL, f"{k}: {v:.2e}"
 
 
-def test_fp32x_graph_replays_match_eager(case_factory):
+@pytest.mark.parametrize("storage", [capi.STORAGE_FP32X, capi.STORAGE_FP32E])
+def test_fp32x_graph_replays_match_eager(case_factory, storage):
     cs = capi.Case(case_factory("shak_quad12_nc24"))
     out = []
     for graphs in (False, True):
-        dev = capi.Device(cs, capi.STORAGE_FP32X)
+        dev = capi.Device(cs, storage)
         dev.set_graphs(graphs)
         dev.init_equilibrium()
         dev.step(6)
@@ -105,10 +119,11 @@ def test_fp32x_graph_replays_match_eager(case_factory):
 
 
 def test_fp32x_rejects_per_cell_kernel_cases(case_factory):
-    """FP32X runs on the patch kernels only: a 36-velocity set is refused."""
+    """FP32X and FP32E run on the patch kernels only: a 36-velocity set is refused."""
     cs = capi.Case(case_factory("shak_quad6_nc6_lim"))
-    with pytest.raises(capi.MesoError):
-        capi.Device(cs, capi.STORAGE_FP32X)
+    for st in (capi.STORAGE_FP32X, capi.STORAGE_FP32E):
+        with pytest.raises(capi.MesoError):
+            capi.Device(cs, st)
 
 
 def test_fp32x_closer_than_fp32(case_factory):
@@ -154,7 +169,7 @@ def test_fp32_and_fp32x_at_convergence_kn1_cavity(tmp_path):
         assert step <= 20000, "FP64 did not converge"
     _say(f"[gpu] FP64 converged at step {step}")
     mr = ref.macros()
-    for storage, tol in ((capi.STORAGE_FP32, 1e-6), (capi.STORAGE_FP32X, 1e-8)):
+    for storage, tol in ((capi.STORAGE_FP32, 1e-6), (capi.STORAGE_FP32E, 1e-6), (capi.STORAGE_FP32X, 1e-8)):
         dev = capi.Device(cs, storage)
         dev.init_equilibrium()
         rep = dev.step(step)

@@ -97,3 +97,28 @@ def test_runner_matches_meso_output(name, tmp_path):
             assert text == "", f  # meso's flux fields are empty before the first step
         else:
             assert text == open(os.path.join(saved, f)).read(), f
+
+
+@pytest.mark.parametrize("storage,tol", [("fp32", 1e-6), ("fp32e", 1e-6), ("fp32x", 1e-7)])
+def test_runner_reduced_storages(storage, tol, tmp_path):
+    """--storage fp32 / fp32e / fp32x on meso's Shakhov quad case (64 velocities): the
+    same files as meso's, every number within the storage's bound of them."""
+    fx = os.path.join(GOLDEN, "out_shak_quad6")
+    for f in os.listdir(fx):
+        if os.path.isfile(os.path.join(fx, f)):
+            shutil.copy(os.path.join(fx, f), tmp_path / f)
+    case = str(tmp_path / "case.txt")
+    set_keys(case, max_step=2, write_interval=2)
+    run(["--case", case, "--out", str(tmp_path), "--storage", storage], tmp_path)
+    case_name = [l.split()[1] for l in open(case) if l.startswith("case-name")][0]
+    ref, mine = os.path.join(fx, "out", case_name), str(tmp_path / case_name)
+    assert tree(mine) == tree(ref)
+    for f in tree(ref):
+        if f.endswith("grid.plt"):
+            continue
+        wa, na = split(os.path.join(mine, f))
+        wb, nb = split(os.path.join(ref, f))
+        assert wa == wb and na.shape == nb.shape, f
+        if nb.size:
+            err = np.abs(na - nb).max() / max(np.abs(nb).max(), 1e-6)
+            assert err < tol, f"{storage} {f}: {err:.2e}"
