@@ -221,6 +221,59 @@ def macro_errors(a, b, dim, shakhov):
     return e
 
 
+def converge_one(args):
+    """One storage of the workload stepped alone (at --scale): to meso's
+    stopping rule (every 100 steps, ten checks in a row with every residual
+    <= the case's residual-limit) or for --converge-steps steps; the final
+    macros and the residual history go to --out."""
+    import numpy as np
+    work = os.path.join(tempfile.gettempdir(), "meso_hip_bench")
+    cs = capi.Case(configs.build(args.workload, work, args.scale))
+    dim = cs.mesh.dim
+    limit = cs.setting_double("residual-limit", 1e-6)
+    idx = [0, 4, 1, 2, 5, 6] if dim == 2 else [0, 4, 1, 2, 3, 5, 6, 7]
+    st = {"fp64": capi.STORAGE_FP64, "fp32": capi.STORAGE_FP32, "fp32e": capi.STORAGE_FP32E,
+          "fp32x": capi.STORAGE_FP32X}[args.converge_one]
+    dev = capi.Device(cs, st)
+    dev.init_equilibrium()
+    dev.residual()
+    t0, step, run, hist = time.time(), 0, 0, []
+    while step < args.max_steps and time.time() - t0 < args.time_limit:
+        rep = dev.step(100)
+        if rep.bad_cells:
+            raise RuntimeError(f"numerical failure at step {step + rep.steps_done}")
+        step += 100
+        r = np.asarray(dev.residual())[idx]
+        hist.append(float(r.max()))
+        run = run + 1 if bool(np.all(r <= limit)) else 0
+        if step % 1000 == 0:
+            log(f"[converge-one] {args.converge_one} step {step} residual {r.max():.3e} ({run}) t={time.time() - t0:.0f}s")
+        if (args.converge_steps == 0 and run >= 10) or (args.converge_steps and step >= args.converge_steps):
+            break
+    m = dev.macros()
+    if args.out:
+        np.savez(args.out, step=step, rho=m["rho"], u=m["u"], T=m["T"], q=m["q"], residual=np.array(hist), dim=dim,
+                 shakhov=int(cs.model == capi.MODEL_CDUGKS_SHAKHOV))
+    print(json.dumps({"storage": args.converge_one, "workload": args.workload, "mesh_scale": args.scale,
+                      "steps": step, "converged": run >= 10, "seconds": time.time() - t0,
+                      "last_residual": hist[-1] if hist else None}), flush=True)
+
+
+def converge_compare(paths):
+    """Relative L2 over cells of every macro of each npz against the first."""
+    import numpy as np
+    ref = np.load(paths[0])
+    out = {"reference": os.path.basename(paths[0]), "steps": int(ref["step"]), "errors": {}}
+    for p in paths[1:]:
+        b = np.load(p)
+        if int(b["step"]) != int(ref["step"]):
+            raise SystemExit(f"{p}: {int(b['step'])} steps against {int(ref['step'])}")
+        out["errors"][os.path.basename(p)] = macro_errors(
+            {"rho": b["rho"], "u": b["u"], "T": b["T"], "q": b["q"]},
+            {"rho": ref["rho"], "u": ref["u"], "T": ref["T"], "q": ref["q"]}, int(ref["dim"]), bool(ref["shakhov"]))
+    print(json.dumps(out), flush=True)
+
+
 def converge_main(args):
     line = converge_run(args.workload, args.scale, args.max_steps, args.time_limit, args.residual_limit,
                         args.curve_every, args.run_on)
@@ -365,9 +418,23 @@ def main():
     ap.add_argument("--curve-every", type=int, default=1000, help="--converge: steps between error samples")
     ap.add_argument("--run-on", action="store_true",
                     help="--converge: keep stepping FP32 / FP32X after FP64 converged until they do")
+    ap.add_argument("--converge-one", choices=("fp64", "fp32", "fp32e", "fp32x"), default=None,
+                    help="instead: run one storage alone (the full-resolution runs do not fit one call together): "
+                         "to meso's convergence rule, or for --converge-steps steps; its macros go to --out (npz)")
+    ap.add_argument("--converge-steps", type=int, default=0, help="--converge-one: steps (0: to convergence)")
+    ap.add_argument("--out", default="", help="--converge-one: npz file of the final macros")
+    ap.add_argument("--converge-compare", nargs="+", default=None, metavar="NPZ",
+                    help="instead: relative L2 of every macro of the npz files (after the first) against the "
+                         "first (the FP64 run), as one JSON line")
     args = ap.parse_args()
     if args.converge:
         converge_main(args)
+        return
+    if args.converge_one:
+        converge_one(args)
+        return
+    if args.converge_compare:
+        converge_compare(args.converge_compare)
         return
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -12,7 +12,7 @@
 // and specialises handle_solver<T> with the loop of exec.cpp:141-165.
 //
 // One class serves the three solvers; the case key `solver` picks the model
-// (src/meso/main.cpp:47-57). The case key `storage` (fp64 | fp32 | fp32x) picks
+// (src/meso/main.cpp:47-57). The case key `storage` (fp64 | fp32 | fp32e | fp32x) picks
 // the device storage of the distributions; fp64 is meso's own arithmetic.
 //
 // A maintainer adds to meso:
@@ -57,7 +57,8 @@ public:
         int st = MESO_STORAGE_FP64;
         if (storage == "fp32") st = MESO_STORAGE_FP32;
         else if (storage == "fp32x") st = MESO_STORAGE_FP32X;
-        else if (storage != "fp64") fail("storage must be fp64, fp32 or fp32x, not " + storage);
+        else if (storage == "fp32e") st = MESO_STORAGE_FP32E;
+        else if (storage != "fp64") fail("storage must be fp64, fp32, fp32e or fp32x, not " + storage);
         if (meso_hip_create(&m, &d, &params_, st, &ctx_) != MESO_OK) {
             const std::string e = meso_hip_last_error(nullptr);
             meso_case_close(cs_);
