@@ -201,50 +201,6 @@ __device__ __forceinline__ void stage_values_x(const KArgs &a, const int *rows, 
     }
 }
 
-// FP32E storage (FP32 kernels on the FP32X format): the rows hold the FP32
-// word and extension byte of the deviation from f_ref; they are staged as FP32
-// values stochastically rounded from the 31-bit deviation, so the gradients
-// and face values see the state with an unbiased FP32 rounding each step while
-// the state itself keeps 31 bits. The word is the value truncated toward zero
-// and the byte the next 8 bits of its significand, so the stochastic rounding
-// is the word moved one unit away from zero with probability byte / 256 (a
-// random byte below it): integer operations only (r: one hashed word per lane
-// and tile, stepped per row).
-template <int ND, int VT>
-__device__ __forceinline__ void stage_values_xf(const KArgs &a, const int *rows, int nr, int nvp, int v, bool ok,
-                                                int slot, int vl, float *val, int maxr, unsigned r) {
-    constexpr int NSLOT = (BLOCKP / 64) * (64 / VT);
-    constexpr int B = 4;  // rows per batch (8 raised the moment kernel past the registers of 7 waves per SIMD: 8.3 ms against 6.5)
-    const float *wg = (const float *)a.fin_g, *wh = (const float *)a.fin_h;
-    const unsigned *xg = a.xin_g.as<const unsigned>(), *xh = a.xin_h.as<const unsigned>();
-    for (int base = slot; base < nr; base += NSLOT * B) {
-        float w[B][ND];
-        unsigned e[B][ND];
-#pragma unroll
-        for (int k = 0; k < B; ++k) {
-            const int rr = base + k * NSLOT;
-            const bool on = ok && rr < nr;
-            const size_t o = size_t(on ? rows[rr] : 0) * nvp + (on ? v : 0);
-#pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                w[k][d] = on ? (d == 0 ? wg : wh)[o] : 0.0f;
-                e[k][d] = on ? (d == 0 ? xg : xh)[o >> 2] >> (8 * (o & 3)) : 0u;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < B; ++k) {
-            const int rr = base + k * NSLOT;
-            const unsigned rk = r + unsigned(base + k * NSLOT) * 0x9e3779b9u;
-            if (rr < nr)
-#pragma unroll
-                for (int d = 0; d < ND; ++d) {
-                    const unsigned up = ((rk >> (8 * d)) & 0xffu) < (e[k][d] & 0xffu) ? 1u : 0u;
-                    val[(d * maxr + rr) * VT + vl] = ok ? __uint_as_float(__float_as_uint(w[k][d]) + up) : 0.0f;
-                }
-        }
-    }
-}
-
 // Patch records: with one item per wave (IPW == 1) every record a wave reads
 // is wave-uniform and is read from HBM through the scalar cache; with several
 // items per wave the records are staged in LDS.
@@ -558,10 +514,9 @@ __device__ __forceinline__ void d_patch_moments(const KArgs &a, const int b) {
             ref[0] = ok ? a.fref_d[v] : 0.0;
             ref[1] = ok && SH ? a.fref_d[a.s.nv + v] : 0.0;
             stage_values_x<ND, VT>(a, rows, nr, nvp, v, ok, ref, slot, vl, (double *)val, maxr);
-        } else if constexpr (XS) {
-            stage_values_xf<ND, VT>(a, rows, nr, nvp, v, ok, slot, vl, (float *)val, maxr,
-                                    mix32(fseed ^ 0x3c6ef372u ^ (unsigned(t) << 20) ^ unsigned(b * BLOCKP + int(threadIdx.x))));
         } else {
+            // FP32 and FP32E storage: the rows' FP32 words (with FP32E, the
+            // words are the 31-bit deviations rounded to nearest FP32)
             stage_values<ND, S, VT>(rows, nr, fin_g, fin_h, nvp, v, ok, slot, vl, val, maxr);
         }
         if (threadIdx.x < VT) {

@@ -2294,22 +2294,23 @@ int step_patch(meso_hip_ctx *c, KArgs a) {
     constexpr int VTM = sizeof(T) == 4 ? 64 : 32;  // velocities per tile: moments
     constexpr int VTU = upd_vt<T, D>();             // velocities per tile: update
     constexpr int MAXF = D == 2 ? 4 : 6;
+    constexpr bool XM = X && sizeof(T) == 8;  // FP32E moments read the words only (the FP32 kernel)
     const int fwblk = (s.nf + WPB - 1) / WPB;
     if (c->mp_cells) {
         if (!c->mp_attr[1][LIM] && c->lds_mp > 65536) {
-            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, true, X>),
+            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, true, XM>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_mp)));
             c->mp_attr[1][LIM] = true;
         }
-        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, true, X>), c->n_mp * c->p_chunks, c->lds_mp,
+        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, true, XM>), c->n_mp * c->p_chunks, c->lds_mp,
                  a);
     } else {
         if (!c->mp_attr[0][LIM] && c->lds_mp > 65536) {
-            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, false, X>),
+            HIP_TRY(c, hipFuncSetAttribute(reinterpret_cast<const void *>(&k_patch_moments<MODEL, T, VTM, D, LIM, false, XM>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, int(c->lds_mp)));
             c->mp_attr[0][LIM] = true;
         }
-        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, false, X>), c->n_mp * c->p_chunks,
+        LAUNCH_P(c, "face_moments", (k_patch_moments<MODEL, T, VTM, D, LIM, false, XM>), c->n_mp * c->p_chunks,
                  c->lds_mp, a);
     }
     if (MODEL == CDUGKS_SHAKHOV && TAB && face_sep(c))
@@ -4148,15 +4149,16 @@ template <int MODEL, class T, bool TAB, int D, bool LIM, bool X>
 int batch_group_step(meso_hip_batch *b, const BatchGroup &g, int phase, bool cells) {
     constexpr int VTM = sizeof(T) == 4 ? 64 : 32;
     constexpr int VTU = upd_vt<T, D>();
+    constexpr bool XM = X && sizeof(T) == 8;  // FP32E moments read the words only (the FP32 kernel)
     const KArgs *tab = g.tab[phase];
     const hipStream_t st = sub_stream(b, g.sub);
     int rc;
     if (cells) {
-        auto *km = &kb_patch_moments<MODEL, T, VTM, D, LIM, true, X>;
+        auto *km = &kb_patch_moments<MODEL, T, VTM, D, LIM, true, XM>;
         if ((rc = batch_lds_attr(b, km, g.lds_mp))) return rc;
         BLAUNCH(st, km, g.blocks[BS_MOM], BLOCKP, g.lds_mp, tab, g.map[BS_MOM]);
     } else {
-        auto *km = &kb_patch_moments<MODEL, T, VTM, D, LIM, false, X>;
+        auto *km = &kb_patch_moments<MODEL, T, VTM, D, LIM, false, XM>;
         if ((rc = batch_lds_attr(b, km, g.lds_mp))) return rc;
         BLAUNCH(st, km, g.blocks[BS_MOM], BLOCKP, g.lds_mp, tab, g.map[BS_MOM]);
     }

@@ -32,14 +32,22 @@ enum Bc : int {
 };
 
 // FP32X storage: a value is an FP32 word plus one byte that extends its
-// significand by 8 bits. The FP64 value is rounded to nearest at 31 explicit
-// mantissa bits; the word holds the upper 23 of them (truncated, so the word
-// converted to FP64 has zeros below) and the byte the next 8. Decoding puts
-// the byte back into those bits of the FP64 pattern: one shift-or.
+// significand by 8 bits. The FP64 value is first rounded at 31 explicit
+// mantissa bits; in units of 2^21 of its FP64 pattern that is an integer q.
+// The word is q rounded to nearest at 23 explicit bits (a multiple of 256 in
+// those units, so the word converted to FP64 has zeros below), and the byte
+// is the remainder q - word, a signed value in [-128, 127]. Decoding adds the
+// sign-extended byte to the word's FP64 pattern: one add. The word alone is
+// the value rounded to nearest FP32, so a kernel that needs only FP32 values
+// reads the words and not the bytes.
+__device__ __forceinline__ void x32_pack(unsigned long long b, float &hi, unsigned char &ext) {
+    const unsigned long long q = b >> 21;
+    const unsigned long long w = (q + 128u) & ~0xffull;
+    ext = (unsigned char)((q - w) & 0xffu);
+    hi = float(__longlong_as_double((long long)(w << 21)));  // exact
+}
 __device__ __forceinline__ void x32_encode(double v, float &hi, unsigned char &ext) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v) + (1ull << 20);
-    ext = (unsigned char)((b >> 21) & 0xffu);
-    hi = float(__longlong_as_double((long long)(b & ~((1ull << 29) - 1))));  // exact
+    x32_pack((unsigned long long)__double_as_longlong(v) + (1ull << 20), hi, ext);
 }
 // The same with stochastic rounding at bit 31: r holds 21 random bits, so the
 // value rounds up with the probability of its position between the two
@@ -47,9 +55,7 @@ __device__ __forceinline__ void x32_encode(double v, float &hi, unsigned char &e
 // its steady state are not lost the way round-to-nearest loses the ones below
 // half a unit in the last place.
 __device__ __forceinline__ void x32_encode_sr(double v, unsigned r, float &hi, unsigned char &ext) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v) + (r & ((1u << 21) - 1u));
-    ext = (unsigned char)((b >> 21) & 0xffu);
-    hi = float(__longlong_as_double((long long)(b & ~((1ull << 29) - 1))));
+    x32_pack((unsigned long long)__double_as_longlong(v) + (r & ((1u << 21) - 1u)), hi, ext);
 }
 // 32-bit mixing function (xorshift-multiply; "lowbias32" constants): the
 // random bits of the stochastic rounding, from (entry index, step seed).
@@ -73,9 +79,10 @@ __device__ __forceinline__ float f32_sr(double v, unsigned r) {
 }
 // Second random word from a first (the other distribution of the same entry).
 __device__ __forceinline__ unsigned rot16(unsigned x) { return (x << 16) | (x >> 16); }
+// The low byte of ext is the signed remainder.
 __device__ __forceinline__ double x32_decode(float hi, unsigned ext) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(double(hi));
-    return __longlong_as_double((long long)(b | ((unsigned long long)(ext & 0xffu) << 21)));
+    const long long b = __double_as_longlong(double(hi));
+    return __longlong_as_double(b + (long long)(signed char)(ext & 0xffu) * (1ll << 21));
 }
 
 // Per-face equilibrium description, written by the face-macro kernel.
