@@ -948,6 +948,14 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
                     ta[q][2] = D == 3 ? tb[q * a.nt + ((L.vt >> 20) & 1023)] : S(0);
                 }
             }
+            // FP32E: the dwords holding the cell's extension bytes (shared by
+            // four lanes), read here for the same reason
+            unsigned xe[2] = {0u, 0u};
+            if constexpr (XS && sizeof(S) == 4) {
+                const size_t o = size_t(c) * nvp + (L.ok ? v : 0);
+                xe[0] = a.xin_g.template as<const unsigned>()[o >> 2] >> (8 * (o & 3));
+                if (SH) xe[1] = a.xin_h.template as<const unsigned>()[o >> 2] >> (8 * (o & 3));
+            }
             double F[2] = {0.0, 0.0};
 #pragma unroll
             for (int k = 0; k < meso_patch::Lim<D>::MAXF; ++k) {
@@ -966,13 +974,9 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
             double fcg = double(val[s * VT + vl]);
             double fch = SH ? double(val[(maxr + s) * VT + vl]) : 0.0;
             if constexpr (XS && sizeof(S) == 4) {
-                // FP32E: the staged word and its extension byte (the dword
-                // that holds it, shared by four lanes)
-                const size_t o = size_t(c) * nvp + (L.ok ? v : 0);
-                fcg = x32_decode(val[s * VT + vl], a.xin_g.template as<const unsigned>()[o >> 2] >> (8 * (o & 3)));
-                if (SH)
-                    fch = x32_decode(val[(maxr + s) * VT + vl],
-                                     a.xin_h.template as<const unsigned>()[o >> 2] >> (8 * (o & 3)));
+                // FP32E: the staged word and its extension byte
+                fcg = x32_decode(val[s * VT + vl], xe[0]);
+                if (SH) fch = x32_decode(val[(maxr + s) * VT + vl], xe[1]);
             }
             double og, oh = 0.0;
             if (SH) {
