@@ -925,8 +925,8 @@ __device__ __forceinline__ void d_patch_update(const KArgs &a, const int b) {
         // C: owned cells (this thread's are slot + j NSLOT, the same in every
         // tile); FP32 2D two cells per step, so that the second cell's table
         // and LDS reads overlap the first cell's arithmetic (FP64 and 3D:
-        // slower)
-        constexpr int CUNR = sizeof(S) == 4 && D == 2 ? 2 : 1;
+        // slower; FP32E: one, as two spill registers: 7.07 against 6.68 ms)
+        constexpr int CUNR = sizeof(S) == 4 && D == 2 && !XS ? 2 : 1;
         // FP32: random bits of the stored state's rounding, one hashed word per
         // lane and tile, stepped by a Weyl increment per owned cell
         const unsigned rs = mix32(seed ^ (unsigned(t) << 20) ^ unsigned(b * BLOCKP + int(threadIdx.x)) ^ 0x2545f491u);
