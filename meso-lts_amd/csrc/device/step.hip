@@ -2512,13 +2512,14 @@ int copy_residual_base(meso_hip_ctx *c) {
     return MESO_OK;
 }
 
-// Velocities per block of the patch kernels (a multiple of 64). At most 2560:
-// the records a block stages once are amortised over its chunk's tiles; and
-// enough chunks that the smaller patch grid has 16 blocks per CU, so that small
-// cases fill the GPU (as far as 64-velocity chunks allow).
+// Velocities per block of the patch kernels (a multiple of 64). At most 1280
+// (the records a block stages once are amortised over its chunk's tiles;
+// 200^2 x 101^2 cavity, every storage: 1-1.7% faster per step than 2560);
+// and enough chunks that the smaller patch grid has 16 blocks per CU, so that
+// small cases fill the GPU (as far as 64-velocity chunks allow).
 int default_vchunk(int nv, int npatch, int ncu) {
     const int most = (nv + 63) / 64;
-    int chunks = std::max((nv + 2559) / 2560, (16 * ncu + npatch - 1) / std::max(npatch, 1));
+    int chunks = std::max((nv + 1279) / 1280, (16 * ncu + npatch - 1) / std::max(npatch, 1));
     chunks = std::max(1, std::min(chunks, most));
     return ((nv + chunks - 1) / chunks + 63) / 64 * 64;
 }
