@@ -77,15 +77,17 @@ def test_fp32x_storage_close_to_oracle(case_factory, name, storage, tol):
         assert v < tol, f"{name} {k}: {v:.2e}"
 
 
-def test_fp32x_state_roundtrip(case_factory):
-    """set_state / get_state keep 31 mantissa bits of the deviation from f_ref."""
+@pytest.mark.parametrize("storage", [capi.STORAGE_FP32X, capi.STORAGE_FP32E])
+def test_fp32x_state_roundtrip(case_factory, storage):
+    """set_state / get_state keep 31 mantissa bits of the deviation from f_ref
+    (FP32X and FP32E store the same format)."""
     cs = capi.Case(case_factory("shak_kn1_quad10"))
     orc = Oracle(cs)
     orc.init_equilibrium()
     orc.step(5)
     f = orc.get_state()
     m = orc.macros()
-    dev = capi.Device(cs, capi.STORAGE_FP32X)
+    dev = capi.Device(cs, storage)
     dev.set_state(f, m["rho"], m["u"], m["T"], m["q"])
     back = dev.get_state()
     # the deviation from the rest Maxwellian is at most ~f, so 2^-31 of f bounds the error
@@ -97,8 +99,9 @@ def test_fp32x_state_roundtrip(case_factory):
     dev.step(5)
     orc.step(5)
     err = macro_errors(cs, dev.macros(), orc.macros())
+    tol = FP32X_TOL if storage == capi.STORAGE_FP32X else FP32E_TOL
     for k, v in err.items():
-        assert v < FP32X_TOL, f"{k}: {v:.2e}"
+        assert v < tol, f"{k}: {v:.2e}"
 
 
 @pytest.mark.parametrize("storage", [capi.STORAGE_FP32X, capi.STORAGE_FP32E])
